@@ -1,0 +1,249 @@
+"""Benchmark of the MI355X LPA-ICI denoiser.
+
+Headline (one JSON line on stdout): plain form, one 4096x4096 fp32 synthetic
+image (sigma 20/255, gamma 2, 16x7 symmetric m=(1,0) kernel set), Mpixel/s
+over K timed forward passes with the input resident in HBM.  The same line
+carries
+  roofline      useful-tap FP32 FLOP/s of the forward kernel (HIP events on
+                its stream) against the MI355X vector FP32 peak
+  cpu_baseline  the reference's own v1_0 CPU code (oracle/_ref, 1 core) on a
+                bounded crop of the same input, or our restatement if the
+                reference build is absent
+  gaussian_m20  the same image with the 16x7 m=(2,0) gaussian-window set
+  batched_uneven  256 images with sides U{128..2048}, per-image sigma/gamma:
+                plain (one launch per image) vs one persistent launch vs its
+                hipGraph replay
+  fwd_bwd       64 x 512^2 forward + adjoint (dL/dx, dL/dkernel)
+Flags --skip-* drop the side legs; --config5 adds 32 images of 1024..8192 px
+fwd+bwd with peak HBM use.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "lpa-ici-cuda_amd"))
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+import lpa_ici  # noqa: E402
+from lpa_ici.metrics import MI355X_FP32_PEAK_TFLOPS, useful_taps  # noqa: E402
+
+METRIC = "Mpixel/s at 4096^2 plain vs batched-uneven; fwd+bwd Mpixel/s; % FP32 VALU peak"
+SIGMA, GAMMA, SEED = 20.0 / 255.0, 2.0, 20250910
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup, stream):
+    """Median-free total: W untimed calls, then K calls between two events on
+    `stream`, synchronized on both sides.  Returns ms per call."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def make_image(w, h, seed, sigma, noise_seed):
+    return lpa_ici.make_noisy(lpa_ici.synth_image(w, h, seed), sigma, noise_seed)
+
+
+def plain_leg(ks, x_host, steps, warmup, stream, label):
+    x = torch.from_numpy(x_host).cuda()
+    y = torch.empty_like(x)
+    maps = torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda")
+    plan = lpa_ici.Plan(ks, [x], [y], [maps], SIGMA, GAMMA, flags=lpa_ici.FWD_PLAIN)
+    with torch.cuda.stream(stream):
+        ms = timed(lambda: plan.run(stream), steps, warmup, stream)
+    taps = useful_taps(maps, ks.runs()["nnz"], ks.S)
+    npx = x.numel()
+    tflops = 2.0 * taps / (ms * 1e-3) / 1e12
+    log(f"[{label}] {x.shape[1]}x{x.shape[0]}: {ms:.3f} ms/pass, {npx / ms / 1e3:.1f} Mpx/s, "
+        f"{taps / npx:.0f} useful taps/px, {tflops:.2f} TFLOP/s")
+    plan.close()
+    return {"ms": ms, "mpx_s": npx / ms / 1e3, "useful_taps": taps, "taps_per_px": taps / npx, "tflops": tflops,
+            "y": y, "x": x}
+
+
+def batched_leg(ks, steps, warmup, stream, n_images=256, seed=3):
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(128, 2049, size=(n_images, 2))
+    sig = [float(rng.choice([10, 20, 30, 40])) / 255.0 for _ in range(n_images)]
+    gam = [float(rng.choice([1.5, 2.0, 2.5])) for _ in range(n_images)]
+    t0 = time.time()
+    xs = [torch.from_numpy(make_image(int(w), int(h), 1000 + i, s, SEED + i)).cuda()
+          for i, ((w, h), s) in enumerate(zip(sides, sig))]
+    log(f"[batched] generated {n_images} images, {sum(x.numel() for x in xs) / 1e6:.1f} Mpx in {time.time() - t0:.1f} s")
+    npx = sum(x.numel() for x in xs)
+    ys_plain = [torch.empty_like(x) for x in xs]
+    ys_batch = [torch.empty_like(x) for x in xs]
+    ys_graph = [torch.empty_like(x) for x in xs]
+    out = {"images": n_images, "mpx": npx / 1e6}
+    plans = {
+        "plain": lpa_ici.Plan(ks, xs, ys_plain, None, sig, gam, flags=lpa_ici.FWD_PLAIN),
+        "batched": lpa_ici.Plan(ks, xs, ys_batch, None, sig, gam, flags=0),
+        "graph": lpa_ici.Plan(ks, xs, ys_graph, None, sig, gam, flags=lpa_ici.FWD_GRAPH),
+    }
+    with torch.cuda.stream(stream):
+        for name, plan in plans.items():
+            ms = timed(lambda: plan.run(stream), steps, warmup, stream)
+            out[name] = {"ms": ms, "mpx_s": npx / ms / 1e3}
+            log(f"[batched] {name}: {ms:.2f} ms, {npx / ms / 1e3:.1f} Mpx/s")
+    torch.cuda.synchronize()
+    out["identical"] = all(torch.equal(a, b) and torch.equal(a, c) for a, b, c in zip(ys_plain, ys_batch, ys_graph))
+    out["batched_over_plain"] = out["batched"]["mpx_s"] / out["plain"]["mpx_s"]
+    out["graph_over_plain"] = out["graph"]["mpx_s"] / out["plain"]["mpx_s"]
+    for p in plans.values():
+        p.close()
+    return out
+
+
+def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
+    xs = [torch.from_numpy(make_image(side, side, 2000 + i, SIGMA, SEED + i)).cuda() for i in range(n)]
+    ys = [torch.empty_like(x) for x in xs]
+    maps = [torch.empty((ks.D, side, side), dtype=torch.uint8, device="cuda") for _ in xs]
+    g = torch.Generator(device="cuda").manual_seed(5)
+    dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
+    dxs = [torch.empty_like(x) for x in xs]
+    nw = ks.info()["num_weights"]
+    dk = torch.zeros(nw, dtype=torch.float32, device="cuda")
+    fwd = lpa_ici.Plan(ks, xs, ys, maps, SIGMA, GAMMA, flags=0)
+    import ctypes
+    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, SIGMA, GAMMA) for x, y, m in zip(xs, ys, maps)])
+    dy_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dys])
+    dx_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dxs])
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    def bwd():
+        lpa_ici.check(lpa_ici.lib().lpa_ici_backward(ks.handle, arr, dy_p, dx_p, ctypes.c_void_p(dk.data_ptr()), n,
+                                                     sp))
+
+    def step():
+        fwd.run(stream)
+        bwd()
+
+    with torch.cuda.stream(stream):
+        ms_f = timed(lambda: fwd.run(stream), steps, warmup, stream)
+        ms_fb = timed(step, steps, warmup, stream)
+    npx = n * side * side
+    log(f"[fwd+bwd] {n}x{side}^2: fwd {ms_f:.2f} ms, fwd+bwd {ms_fb:.2f} ms, {npx / ms_fb / 1e3:.1f} Mpx/s")
+    fwd.close()
+    return {"images": n, "side": side, "fwd_ms": ms_f, "fwd_bwd_ms": ms_fb, "fwd_mpx_s": npx / ms_f / 1e3,
+            "fwd_bwd_mpx_s": npx / ms_fb / 1e3}
+
+
+def cpu_baseline(x_host, crop=256):
+    """The reference's v1_0 CPU code (fastest of its two CPU versions) on a
+    crop of the workload image, one core; our restatement if the reference
+    build is not present."""
+    sys.path.insert(0, REPO)
+    import oracle
+    ks = lpa_ici.load_kernels()
+    var = ks.variances(SIGMA)
+    h, w = x_host.shape
+    sample = np.ascontiguousarray(x_host[(h - crop) // 2:(h + crop) // 2, (w - crop) // 2:(w + crop) // 2])
+    kind = "reference" if oracle.ref_available("v1_0") else "port"
+    t = time.time()
+    if kind == "reference":
+        oracle.ref_denoise(sample, ks, var, GAMMA, version="v1_0")
+    else:
+        oracle.v1_0(sample, ks, var, GAMMA)
+    sec = time.time() - t
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": sample.size / sec / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": kind,
+            "sample": f"v1_0 CPU path on the central {crop}x{crop} crop of the 4096^2 noisy image, "
+                      f"{sec:.1f} s; host {cpu}, nproc {os.cpu_count()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-gaussian", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-fwdbwd", action="store_true")
+    ap.add_argument("--tile-config", type=int, default=-1)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the denoiser has no multi-GPU path in this build "
+              "(images shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    torch.cuda.init()
+    stream = torch.cuda.Stream()
+    ks = lpa_ici.load_kernels()
+    ks.set_tile_config(args.tile_config)
+    t0 = time.time()
+    x_host = make_image(args.size, args.size, 42, SIGMA, SEED)
+    log(f"[setup] {args.size}^2 image in {time.time() - t0:.1f} s; device {torch.cuda.get_device_name(0)}")
+
+    main_leg = plain_leg(ks, x_host, args.steps, args.warmup, stream, "plain symmetric")
+    npx = args.size * args.size
+    rec = {
+        "metric": METRIC,
+        "value": round(main_leg["mpx_s"], 2),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_leg["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain {args.size}x{args.size} fp32 synthetic (polygons, disks, ramps, stripes), "
+                               "sigma 20/255, gamma 2, 16x7 symmetric m=(1,0) set, one launch per pass",
+                   "pixels": npx, "tile_config": ks.tile_config()},
+        "roofline": {"bound": "valu", "achieved": round(main_leg["tflops"], 3), "peak": MI355X_FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(main_leg["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4),
+                     "traffic": None,
+                     "algorithmic_flop_per_launch": 2 * main_leg["useful_taps"],
+                     "useful_taps_per_px": round(main_leg["taps_per_px"], 1)},
+    }
+    if not args.skip_cpu:
+        rec["cpu_baseline"] = cpu_baseline(x_host)
+        log(f"[cpu] {rec['cpu_baseline']}")
+    if not args.skip_gaussian:
+        ksg = lpa_ici.load_kernels(lpa_ici.GAUSSIAN_M20_KERNELS)
+        ksg.set_tile_config(args.tile_config)
+        g = plain_leg(ksg, x_host, args.steps, args.warmup, stream, "plain m20 gaussian")
+        rec["gaussian_m20"] = {"mpx_s": round(g["mpx_s"], 2), "ms": round(g["ms"], 4),
+                               "tflops": round(g["tflops"], 3), "taps_per_px": round(g["taps_per_px"], 1)}
+    del main_leg
+    if not args.skip_batched:
+        b = batched_leg(ks, max(3, args.steps // 4), 1, stream)
+        rec["batched_uneven"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in b.items()
+                                 if not isinstance(v, dict)}
+        for k in ("plain", "batched", "graph"):
+            rec["batched_uneven"][k] = {kk: round(vv, 3) for kk, vv in b[k].items()}
+    if not args.skip_fwdbwd:
+        fb = fwd_bwd_leg(ks, max(3, args.steps // 4), 1, stream)
+        rec["fwd_bwd"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in fb.items()}
+    rec["host"] = platform.node()
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

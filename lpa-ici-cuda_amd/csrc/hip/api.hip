@@ -1,0 +1,564 @@
+// C ABI over the HIP kernels: kernel sets, plans and the forward pass.
+// See include/lpa_ici.h for the contract of every entry point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../internal.hpp"
+#include "lpa_device.hpp"
+#include "lpa_ici.h"
+
+using lpa::fail;
+
+struct lpa_kset {
+    int D = 0, S = 0, K = 0;
+    int radius = 0;
+    int num_nonzero = 0;
+    std::vector<int4> runs;
+    std::vector<float> weights;
+    std::vector<int2> kern;
+    std::vector<int> nnz;
+    std::vector<float> sumsq;
+    int device = 0;
+    int4 *d_runs = nullptr;
+    float *d_weights = nullptr;
+    int2 *d_kern = nullptr;
+    int shape = -1;  // -1: built-in choice
+    // Adjoint tables (see lpa::DevAdjoint).
+    std::vector<int4> runs_flip;
+    std::vector<float> weights_flip;
+    std::vector<int2> tap_pos, kern_w;
+    std::vector<int> tap_kernel;
+    int4 *d_runs_flip = nullptr;
+    float *d_weights_flip = nullptr;
+    int2 *d_tap_pos = nullptr, *d_kern_w = nullptr;
+    int *d_tap_kernel = nullptr;
+    float *d_sumsq = nullptr;
+    bool uploaded = false;
+    std::mutex upload_mutex;
+    lpa::DevKset dev() const { return lpa::DevKset{d_runs, d_weights, d_kern, D, S}; }
+    lpa::DevAdjoint adjoint() const
+    {
+        return lpa::DevAdjoint{lpa::DevKset{d_runs_flip, d_weights_flip, d_kern, D, S},
+                               d_tap_pos,
+                               d_tap_kernel,
+                               d_kern_w,
+                               d_sumsq,
+                               (int)weights.size()};
+    }
+};
+
+namespace {
+
+constexpr int kDefaultShape = 0;
+
+lpa_status hip_fail(hipError_t e, const char *what)
+{
+    return fail(e == hipErrorOutOfMemory ? LPA_ENOMEM : LPA_EHIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                                                                              \
+    do {                                                                                                           \
+        hipError_t e_ = (expr);                                                                                    \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr);                                                          \
+    } while (0)
+
+int shape_of(const lpa_kset *ks) { return ks->shape >= 0 ? ks->shape : kDefaultShape; }
+
+// Row-run packing: for every kernel row with a nonzero, one run from the
+// first to the last nonzero (interior zeros kept), offsets from the centre
+// (rows/2, cols/2).  Follows build_sparse_kernels,
+// src/v4_1_lpa_ici_2D_sparse_shared.cu:826-876, without its constant-memory
+// caps (8192 weights / 1536 runs / 128 kernels).
+void pack_runs(lpa_kset *ks, const float *dense, const int *offsets, const int *rows, const int *cols)
+{
+    int radius = 0;
+    for (int k = 0; k < ks->K; ++k) {
+        const float *w = dense + offsets[k];
+        const int R = rows[k], C = cols[k];
+        const int first_run = (int)ks->runs.size();
+        int nz = 0;
+        float ss = 0.0f;
+        for (int i = 0; i < R * C; ++i) {
+            ss += w[i] * w[i];  // float, dense row-major: compute_kernel_variances' sum
+            nz += w[i] != 0.0f;
+        }
+        for (int r = 0; r < R; ++r) {
+            int first = -1, last = -1;
+            for (int c = 0; c < C; ++c)
+                if (w[r * C + c] != 0.0f) {
+                    if (first < 0) first = c;
+                    last = c;
+                }
+            if (first < 0) continue;
+            const int dr = r - R / 2, dc = first - C / 2, len = last - first + 1;
+            ks->runs.push_back(int4{dr, dc, len, (int)ks->weights.size()});
+            for (int c = first; c <= last; ++c) ks->weights.push_back(w[r * C + c]);
+            radius = std::max(radius, std::max(std::abs(dr), std::max(std::abs(dc), std::abs(dc + len - 1))));
+        }
+        ks->kern.push_back(int2{first_run, (int)ks->runs.size() - first_run});
+        const int first_w = first_run < (int)ks->runs.size() ? ks->runs[first_run].w : (int)ks->weights.size();
+        ks->kern_w.push_back(int2{first_w, (int)ks->weights.size() - first_w});
+        ks->nnz.push_back(nz);
+        ks->sumsq.push_back(ss);
+        ks->num_nonzero += nz;
+    }
+    ks->radius = radius;
+    // Adjoint tables: tap positions and owners, and the rotated runs.
+    ks->weights_flip.resize(ks->weights.size());
+    ks->tap_pos.resize(ks->weights.size());
+    ks->tap_kernel.resize(ks->weights.size());
+    for (int k = 0; k < ks->K; ++k) {
+        const int2 kr = ks->kern[k];
+        for (int r = kr.x; r < kr.x + kr.y; ++r) {
+            const int4 run = ks->runs[r];
+            ks->runs_flip.push_back(int4{-run.x, -(run.y + run.z - 1), run.z, run.w});
+            for (int j = 0; j < run.z; ++j) {
+                ks->weights_flip[run.w + j] = ks->weights[run.w + run.z - 1 - j];
+                ks->tap_pos[run.w + j] = int2{run.x, run.y + j};
+                ks->tap_kernel[run.w + j] = k;
+            }
+        }
+    }
+}
+
+// Per-image [3][K] table: variance, threshold gamma*sqrt(var) and 1/var, in
+// float on the host exactly as the reference forms them
+// (src/v0_0_lpa_ici_2D_naive.cpp:684-699 and :887, :918-919).
+void image_table(const lpa_kset *ks, const lpa_image &img, float *tab)
+{
+    const int K = ks->K;
+    const float s2 = img.sigma * img.sigma;
+    for (int k = 0; k < K; ++k) {
+        const float var = img.variances ? img.variances[k] : s2 * ks->sumsq[k];
+        tab[k] = var;
+        tab[K + k] = img.gamma * std::sqrt(var);
+        tab[2 * K + k] = 1.0f / var;
+    }
+}
+
+lpa_status check_images(const lpa_kset *ks, const lpa_image *imgs, int n)
+{
+    if (!ks) return fail(LPA_EINVAL, "null kernel set");
+    if (n <= 0 || !imgs) return fail(LPA_EINVAL, "no images");
+    for (int i = 0; i < n; ++i) {
+        const lpa_image &m = imgs[i];
+        const std::string at = "image " + std::to_string(i) + ": ";
+        if (!m.x || !m.y) return fail(LPA_EINVAL, at + "null x or y");
+        if (m.width <= 0 || m.height <= 0) return fail(LPA_EINVAL, at + "width and height must be positive");
+        if (m.width > 65535 || m.height > 65535 || (long long)m.width * m.height > (1LL << 31) - 1)
+            return fail(LPA_ETOOBIG, at + "image too large");
+        if (m.pitch != 0 && m.pitch < m.width) return fail(LPA_EINVAL, at + "pitch smaller than width");
+        if (!m.variances && !(m.sigma > 0.0f)) return fail(LPA_EINVAL, at + "sigma must be positive");
+        if (!(m.gamma > 0.0f)) return fail(LPA_EINVAL, at + "gamma must be positive");
+    }
+    return LPA_OK;
+}
+
+int tiles_x_of(int shape, int W) { return (W + lpa::kShapes[shape].tw - 1) / lpa::kShapes[shape].tw; }
+int tiles_y_of(int shape, int H) { return (H + lpa::kShapes[shape].th - 1) / lpa::kShapes[shape].th; }
+
+int num_cus()
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 256;
+    return cus;
+}
+
+// Device copies of the layout, made on the first call that needs them (on
+// the device current at that call).  Thread-safe.
+lpa_status ensure_uploaded(lpa_kset *ks)
+{
+    std::lock_guard<std::mutex> lock(ks->upload_mutex);
+    if (ks->uploaded) return LPA_OK;
+    const int K = ks->K;
+    hipError_t e = hipGetDevice(&ks->device);
+    const size_t nr = std::max<size_t>(1, ks->runs.size()), nw = std::max<size_t>(1, ks->weights.size());
+    if (e == hipSuccess) e = hipMalloc(&ks->d_runs, sizeof(int4) * nr);
+    // Weights get 4 floats of slack: the 4-tap loop never reads past a run,
+    // but the slack keeps any scalar prefetch of the last run in bounds.
+    if (e == hipSuccess) e = hipMalloc(&ks->d_weights, sizeof(float) * (nw + 4));
+    if (e == hipSuccess) e = hipMalloc(&ks->d_kern, sizeof(int2) * K);
+    if (e == hipSuccess) e = hipMemset(ks->d_weights, 0, sizeof(float) * (nw + 4));
+    if (e == hipSuccess && !ks->runs.empty())
+        e = hipMemcpy(ks->d_runs, ks->runs.data(), sizeof(int4) * ks->runs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ks->weights.empty())
+        e = hipMemcpy(ks->d_weights, ks->weights.data(), sizeof(float) * ks->weights.size(),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ks->d_kern, ks->kern.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&ks->d_runs_flip, sizeof(int4) * nr);
+    if (e == hipSuccess) e = hipMalloc(&ks->d_weights_flip, sizeof(float) * (nw + 4));
+    if (e == hipSuccess) e = hipMalloc(&ks->d_tap_pos, sizeof(int2) * nw);
+    if (e == hipSuccess) e = hipMalloc(&ks->d_tap_kernel, sizeof(int) * nw);
+    if (e == hipSuccess) e = hipMalloc(&ks->d_kern_w, sizeof(int2) * K);
+    if (e == hipSuccess) e = hipMalloc(&ks->d_sumsq, sizeof(float) * K);
+    if (e == hipSuccess) e = hipMemset(ks->d_weights_flip, 0, sizeof(float) * (nw + 4));
+    if (e == hipSuccess && !ks->runs.empty())
+        e = hipMemcpy(ks->d_runs_flip, ks->runs_flip.data(), sizeof(int4) * ks->runs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !ks->weights.empty()) {
+        const size_t n = ks->weights.size();
+        e = hipMemcpy(ks->d_weights_flip, ks->weights_flip.data(), sizeof(float) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(ks->d_tap_pos, ks->tap_pos.data(), sizeof(int2) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(ks->d_tap_kernel, ks->tap_kernel.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemcpy(ks->d_kern_w, ks->kern_w.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ks->d_sumsq, ks->sumsq.data(), sizeof(float) * K, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "kernel set upload");
+    ks->uploaded = true;
+    return LPA_OK;
+}
+
+}  // namespace
+
+// Device state of a batch: per-image descriptors and tables, the tile queue
+// and its counter, in one allocation.
+struct lpa_plan {
+    lpa_kset *ks = nullptr;
+    unsigned flags = 0;
+    int shape = 0;
+    int n = 0;
+    int num_tiles = 0;
+    int grid = 0;
+    std::vector<lpa::ImgDesc> host_desc;  // device pointers, for the plain form
+    void *d_mem = nullptr;
+    lpa::ImgDesc *d_desc = nullptr;
+    uint2 *d_tiles = nullptr;
+    int *d_counter = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Lay out one plan allocation: [counter | descriptors | tables | tiles].
+lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags, bool async, hipStream_t stream,
+                      lpa_plan *p)
+{
+    p->ks = ks;
+    p->flags = flags;
+    p->shape = shape_of(ks);
+    p->n = n;
+    const int K = ks->K;
+    std::vector<uint2> tiles;
+    for (int i = 0; i < n; ++i) {
+        const int tx = tiles_x_of(p->shape, imgs[i].width), ty = tiles_y_of(p->shape, imgs[i].height);
+        for (int y = 0; y < ty; ++y)
+            for (int x = 0; x < tx; ++x) tiles.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
+    }
+    p->num_tiles = (int)tiles.size();
+    const size_t off_desc = 256;
+    const size_t off_tab = off_desc + ((sizeof(lpa::ImgDesc) * n + 255) / 256) * 256;
+    const size_t off_tiles = off_tab + ((sizeof(float) * 3 * K * n + 255) / 256) * 256;
+    const size_t bytes = off_tiles + sizeof(uint2) * tiles.size();
+    std::vector<unsigned char> host(bytes, 0);
+    if (async) {
+        HIP_TRY(hipMallocAsync(&p->d_mem, bytes, stream));
+    } else {
+        HIP_TRY(hipMalloc(&p->d_mem, bytes));
+    }
+    unsigned char *base = (unsigned char *)p->d_mem;
+    p->d_counter = (int *)base;
+    p->d_desc = (lpa::ImgDesc *)(base + off_desc);
+    p->d_tiles = (uint2 *)(base + off_tiles);
+    float *h_tab = (float *)(host.data() + off_tab);
+    const float *d_tab = (const float *)(base + off_tab);
+    p->host_desc.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const lpa_image &m = imgs[i];
+        image_table(ks, m, h_tab + (size_t)3 * K * i);
+        lpa::ImgDesc &d = p->host_desc[i];
+        d.x = m.x;
+        d.y = m.y;
+        d.map = m.scale_map;
+        d.tab = d_tab + (size_t)3 * K * i;
+        d.W = m.width;
+        d.H = m.height;
+        d.pitch = m.pitch ? m.pitch : m.width;
+        d.tiles_x = tiles_x_of(p->shape, m.width);
+    }
+    std::memcpy(host.data() + off_desc, p->host_desc.data(), sizeof(lpa::ImgDesc) * n);
+    std::memcpy(host.data() + off_tiles, tiles.data(), sizeof(uint2) * tiles.size());
+    HIP_TRY(hipMemcpyAsync(p->d_mem, host.data(), bytes, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));  // `host` goes out of scope on return
+    const int bpc = lpa::blocks_per_cu(p->shape, ks->radius);
+    p->grid = std::max(1, std::min(p->num_tiles, num_cus() * bpc));
+    return LPA_OK;
+}
+
+lpa_status plan_launch(lpa_plan *p, hipStream_t stream)
+{
+    const bool exact = (p->flags & LPA_FWD_EXACT) != 0;
+    const lpa::DevKset dk = p->ks->dev();
+    if (p->flags & LPA_FWD_PLAIN) {
+        for (int i = 0; i < p->n; ++i) {
+            hipError_t e = lpa::launch_forward_plain(p->shape, exact, dk, p->host_desc[i], p->ks->radius, stream);
+            if (e != hipSuccess) return hip_fail(e, "forward launch");
+        }
+    } else {
+        hipError_t e = lpa::launch_forward_batched(p->shape, exact, dk, p->d_desc, p->d_tiles, p->num_tiles,
+                                                   p->d_counter, p->ks->radius, p->grid, stream);
+        if (e != hipSuccess) return hip_fail(e, "batched forward launch");
+    }
+    return LPA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lpa_status lpa_kset_create(int num_dirs, int num_scales, const float *dense, const int *offsets, const int *rows,
+                           const int *cols, lpa_kset_t *out)
+{
+    if (!out) return fail(LPA_EINVAL, "null output handle");
+    *out = nullptr;
+    if (num_dirs <= 0 || num_scales <= 0 || num_scales > 255 || !dense || !offsets || !rows || !cols)
+        return fail(LPA_EINVAL, "bad kernel set arguments");
+    const int K = num_dirs * num_scales;
+    for (int k = 0; k < K; ++k) {
+        if (rows[k] <= 0 || cols[k] <= 0 || offsets[k] < 0)
+            return fail(LPA_EINVAL, "kernel " + std::to_string(k) + " has a bad size or offset");
+        if (rows[k] / 2 > lpa::kMaxRadius || cols[k] / 2 > lpa::kMaxRadius)
+            return fail(LPA_ETOOBIG, "kernel " + std::to_string(k) + " is " + std::to_string(rows[k]) + "x" +
+                                         std::to_string(cols[k]) + ", max supported is " +
+                                         std::to_string(2 * lpa::kMaxRadius + 1) + "x" +
+                                         std::to_string(2 * lpa::kMaxRadius + 1));
+    }
+    lpa_kset *ks = new (std::nothrow) lpa_kset;
+    if (!ks) return fail(LPA_ENOMEM, "out of host memory");
+    ks->D = num_dirs;
+    ks->S = num_scales;
+    ks->K = K;
+    pack_runs(ks, dense, offsets, rows, cols);
+    *out = ks;
+    lpa::clear_error();
+    return LPA_OK;
+}
+
+void lpa_kset_destroy(lpa_kset_t ks)
+{
+    if (!ks) return;
+    if (ks->d_runs) (void)hipFree(ks->d_runs);
+    if (ks->d_weights) (void)hipFree(ks->d_weights);
+    if (ks->d_kern) (void)hipFree(ks->d_kern);
+    for (void *p : {(void *)ks->d_runs_flip, (void *)ks->d_weights_flip, (void *)ks->d_tap_pos,
+                    (void *)ks->d_tap_kernel, (void *)ks->d_kern_w, (void *)ks->d_sumsq})
+        if (p) (void)hipFree(p);
+    delete ks;
+}
+
+lpa_status lpa_kset_get_info(lpa_kset_t ks, lpa_kset_info *info)
+{
+    if (!ks || !info) return fail(LPA_EINVAL, "null argument");
+    info->num_dirs = ks->D;
+    info->num_scales = ks->S;
+    info->num_runs = (int)ks->runs.size();
+    info->num_weights = (int)ks->weights.size();
+    info->num_nonzero = ks->num_nonzero;
+    info->max_radius = ks->radius;
+    return LPA_OK;
+}
+
+lpa_status lpa_kset_get_runs(lpa_kset_t ks, int *run_row, int *run_col, int *run_len, int *run_woff,
+                             int *kernel_first_run, int *kernel_num_runs, float *weights, int *nnz, float *sumsq)
+{
+    if (!ks) return fail(LPA_EINVAL, "null kernel set");
+    for (size_t r = 0; r < ks->runs.size(); ++r) {
+        if (run_row) run_row[r] = ks->runs[r].x;
+        if (run_col) run_col[r] = ks->runs[r].y;
+        if (run_len) run_len[r] = ks->runs[r].z;
+        if (run_woff) run_woff[r] = ks->runs[r].w;
+    }
+    for (int k = 0; k < ks->K; ++k) {
+        if (kernel_first_run) kernel_first_run[k] = ks->kern[k].x;
+        if (kernel_num_runs) kernel_num_runs[k] = ks->kern[k].y;
+        if (nnz) nnz[k] = ks->nnz[k];
+        if (sumsq) sumsq[k] = ks->sumsq[k];
+    }
+    if (weights && !ks->weights.empty()) std::memcpy(weights, ks->weights.data(), sizeof(float) * ks->weights.size());
+    return LPA_OK;
+}
+
+int lpa_num_tile_configs(void) { return lpa::kNumShapes; }
+
+lpa_status lpa_tile_config_get(int index, lpa_tile_config *cfg)
+{
+    if (!cfg || index < 0 || index >= lpa::kNumShapes) return fail(LPA_EINVAL, "bad tile configuration index");
+    cfg->tile_w = lpa::kShapes[index].tw;
+    cfg->tile_h = lpa::kShapes[index].th;
+    cfg->px_per_lane = lpa::kShapes[index].px;
+    cfg->halo = lpa::kMaxRadius;
+    cfg->blocks_per_cu = 0;
+    return LPA_OK;
+}
+
+lpa_status lpa_kset_set_tile_config(lpa_kset_t ks, int index)
+{
+    if (!ks || index < -1 || index >= lpa::kNumShapes) return fail(LPA_EINVAL, "bad tile configuration index");
+    ks->shape = index;
+    return LPA_OK;
+}
+
+int lpa_kset_get_tile_config(lpa_kset_t ks) { return ks ? shape_of(ks) : -1; }
+
+lpa_status lpa_ici_forward(lpa_kset_t ks, const lpa_image *imgs, int n, unsigned flags, void *stream)
+{
+    lpa_status st = check_images(ks, imgs, n);
+    if (st != LPA_OK) return st;
+    if ((st = ensure_uploaded(ks)) != LPA_OK) return st;
+    if (flags & LPA_FWD_GRAPH) return fail(LPA_EINVAL, "LPA_FWD_GRAPH applies to plans only");
+    hipStream_t s = (hipStream_t)stream;
+    lpa_plan p;
+    st = plan_build(ks, imgs, n, flags, true, s, &p);
+    if (st == LPA_OK) st = plan_launch(&p, s);
+    if (p.d_mem) {
+        hipError_t e = hipFreeAsync(p.d_mem, s);
+        if (st == LPA_OK && e != hipSuccess) st = hip_fail(e, "hipFreeAsync");
+    }
+    if (st == LPA_OK) lpa::clear_error();
+    return st;
+}
+
+lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *const *dy, float *const *dx,
+                            float *dkernel, int n, void *stream)
+{
+    lpa_status st = check_images(ks, imgs, n);
+    if (st != LPA_OK) return st;
+    if ((st = ensure_uploaded(ks)) != LPA_OK) return st;
+    if (!dy || !dx) return fail(LPA_EINVAL, "null dy or dx array");
+    if (ks->S > 16) return fail(LPA_ETOOBIG, "the adjoint supports at most 16 scales");
+    for (int i = 0; i < n; ++i) {
+        if (!imgs[i].scale_map) return fail(LPA_EINVAL, "image " + std::to_string(i) + ": the adjoint needs the scale map");
+        if (imgs[i].variances)
+            return fail(LPA_EINVAL, "image " + std::to_string(i) +
+                                        ": the adjoint differentiates sigma^2*sum(w^2); explicit variances are not allowed");
+        if (!dy[i] || !dx[i]) return fail(LPA_EINVAL, "image " + std::to_string(i) + ": null dy or dx");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int K = ks->K, NW = (int)ks->weights.size();
+    // Tiles and tables as for the batched forward, shape 0.
+    std::vector<uint2> tiles;
+    for (int i = 0; i < n; ++i) {
+        const int tx = tiles_x_of(0, imgs[i].width), ty = tiles_y_of(0, imgs[i].height);
+        for (int y = 0; y < ty; ++y)
+            for (int x = 0; x < tx; ++x) tiles.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
+    }
+    int bp = 1, bd = 1;
+    lpa::backward_occupancy(ks->radius, ks->S, K, &bp, &bd);
+    const int cus = num_cus();
+    const int grid_prep = std::max(1, std::min((int)tiles.size(), cus * bp));
+    const int grid_dx = std::max(1, std::min((int)tiles.size(), cus * bd));
+    size_t qpx = 0;
+    for (int i = 0; i < n; ++i) qpx += (size_t)imgs[i].width * imgs[i].height;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t off_desc = 256;
+    const size_t off_tab = off_desc + up(sizeof(lpa::BwdDesc) * n);
+    const size_t off_tiles = off_tab + up(sizeof(float) * 3 * K * n);
+    const size_t off_q = off_tiles + up(sizeof(uint2) * tiles.size());
+    const size_t off_part = off_q + up(sizeof(float) * qpx);
+    const size_t bytes = off_part + sizeof(float) * ((size_t)grid_prep * (NW + K) + K);
+    const size_t host_bytes = off_q;
+    std::vector<unsigned char> host(host_bytes, 0);
+    void *mem = nullptr;
+    HIP_TRY(hipMallocAsync(&mem, bytes, s));
+    unsigned char *base = (unsigned char *)mem;
+    std::vector<lpa::BwdDesc> desc(n);
+    float *h_tab = (float *)(host.data() + off_tab);
+    size_t qoff = 0;
+    for (int i = 0; i < n; ++i) {
+        const lpa_image &m = imgs[i];
+        image_table(ks, m, h_tab + (size_t)3 * K * i);
+        lpa::BwdDesc &b = desc[i];
+        b.img.x = m.x;
+        b.img.y = m.y;
+        b.img.map = m.scale_map;
+        b.img.tab = (const float *)(base + off_tab) + (size_t)3 * K * i;
+        b.img.W = m.width;
+        b.img.H = m.height;
+        b.img.pitch = m.pitch ? m.pitch : m.width;
+        b.img.tiles_x = tiles_x_of(0, m.width);
+        b.g = dy[i];
+        b.dx = dx[i];
+        b.q = (float *)(base + off_q) + qoff;
+        qoff += (size_t)m.width * m.height;
+    }
+    std::memcpy(host.data() + off_desc, desc.data(), sizeof(lpa::BwdDesc) * n);
+    std::memcpy(host.data() + off_tiles, tiles.data(), sizeof(uint2) * tiles.size());
+    hipError_t e = hipMemcpyAsync(mem, host.data(), host_bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
+    if (e == hipSuccess)
+        e = lpa::launch_backward(ks->dev(), ks->adjoint(), (const lpa::BwdDesc *)(base + off_desc),
+                                 (const uint2 *)(base + off_tiles), (int)tiles.size(), (int *)base, ks->radius,
+                                 grid_prep, grid_dx, (float *)(base + off_part), dkernel, s);
+    hipError_t ef = hipFreeAsync(mem, s);
+    if (e != hipSuccess) return hip_fail(e, "backward");
+    if (ef != hipSuccess) return hip_fail(ef, "hipFreeAsync");
+    lpa::clear_error();
+    return LPA_OK;
+}
+
+lpa_status lpa_plan_create(lpa_kset_t ks, const lpa_image *imgs, int n, unsigned flags, lpa_plan_t *out)
+{
+    if (!out) return fail(LPA_EINVAL, "null output handle");
+    *out = nullptr;
+    lpa_status st = check_images(ks, imgs, n);
+    if (st != LPA_OK) return st;
+    if ((st = ensure_uploaded(ks)) != LPA_OK) return st;
+    lpa_plan *p = new (std::nothrow) lpa_plan;
+    if (!p) return fail(LPA_ENOMEM, "out of host memory");
+    st = plan_build(ks, imgs, n, flags, false, nullptr, p);
+    if (st != LPA_OK) {
+        lpa_plan_destroy(p);
+        return st;
+    }
+    *out = p;
+    lpa::clear_error();
+    return LPA_OK;
+}
+
+lpa_status lpa_plan_run(lpa_plan_t p, void *stream)
+{
+    if (!p) return fail(LPA_EINVAL, "null plan");
+    hipStream_t s = (hipStream_t)stream;
+    if (!(p->flags & LPA_FWD_GRAPH)) {
+        lpa_status st = plan_launch(p, s);
+        if (st == LPA_OK) lpa::clear_error();
+        return st;
+    }
+    if (!s) return fail(LPA_EINVAL, "graph replay needs a non-default stream");
+    if (!p->exec) {
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        lpa_status st = plan_launch(p, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (st != LPA_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        p->graph = g;
+        HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(p->exec, s));
+    lpa::clear_error();
+    return LPA_OK;
+}
+
+void lpa_plan_destroy(lpa_plan_t p)
+{
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->d_mem) (void)hipFree(p->d_mem);
+    delete p;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// Device-side data layout shared by the forward and adjoint kernels.
+//
+// Kernel set in HBM (run-length, row runs), built once per set:
+//   runs[r]    = {row offset from the kernel centre, first column offset,
+//                 run length, index of the run's first weight}
+//   weights[]  = the runs' weights, run after run (interior zeros kept)
+//   kern[k]    = {first run, number of runs} for kernel k = d*S + s
+// Per-image tables, [3][K] floats: variance, ICI threshold and 1/variance,
+// each computed on the host in float exactly as the reference does.
+#ifndef LPA_DEVICE_HPP
+#define LPA_DEVICE_HPP
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lpa {
+
+struct DevKset {
+    const int4 *runs;
+    const float *weights;
+    const int2 *kern;
+    int D, S;
+};
+
+// One image of a batch.  tab points at its [3][K] table.
+struct ImgDesc {
+    const float *x;
+    float *y;
+    uint8_t *map;
+    const float *tab;
+    int W, H, pitch;
+    int tiles_x;
+};
+
+// Compiled tile shapes: output tile tw x th pixels, px adjacent pixels per
+// lane along a row, 256 lanes per work-group.
+struct TileShape {
+    int tw, th, px;
+};
+constexpr int kThreads = 256;
+constexpr int kNumShapes = 5;
+constexpr TileShape kShapes[kNumShapes] = {
+    {32, 32, 4},  // 0: wave covers 32x8 px
+    {64, 16, 4},  // 1: wave covers 64x4 px
+    {32, 16, 2},  // 2: wave covers 32x4 px
+    {16, 16, 1},  // 3: one pixel per lane, wave covers 16x4 px
+    {64, 32, 8},  // 4: wave covers 64x8 px
+};
+constexpr int kMaxRadius = 31;
+
+// LDS row pitch for a tile of width tw and halo radius r: at least tw + 2r and
+// congruent to 1 mod 32, so lanes four pixels apart on consecutive rows fall
+// in different banks.
+__host__ __device__ inline int lds_pitch(int tw, int r)
+{
+    int p = tw + 2 * r;
+    return p + ((1 - p) % 32 + 32) % 32;
+}
+__host__ __device__ inline int lds_floats(int tw, int th, int r) { return (th + 2 * r) * lds_pitch(tw, r); }
+
+// One image of an adjoint batch: the forward's image (x, y = the forward's
+// output, map = its scale indices, tab) plus the upstream gradient g, the
+// output dx and a per-pixel workspace q = g / sum_d(1/var_d).
+struct BwdDesc {
+    ImgDesc img;
+    const float *g;
+    float *dx;
+    float *q;
+};
+
+// Extra kernel-set tables the adjoint reads.
+//   flip: the 180-degree rotated runs (-row, -(col + len - 1), len) with
+//         reversed weights, so the transposed correlation reuses correlate()
+//   tap_pos[i] = {row, col} offset of run weight i from the kernel centre
+//   tap_kernel[i] = kernel index k of run weight i
+//   kern_w[k] = {first run weight, number of run weights} of kernel k
+struct DevAdjoint {
+    DevKset flip;
+    const int2 *tap_pos;
+    const int *tap_kernel;
+    const int2 *kern_w;
+    const float *sumsq;
+    int num_weights;
+};
+
+// Launch entry points (lpa_fwd.hip).
+hipError_t launch_forward_plain(int shape, bool exact, const DevKset &ks, const ImgDesc &img, int radius,
+                                hipStream_t stream);
+hipError_t launch_forward_batched(int shape, bool exact, const DevKset &ks, const ImgDesc *d_imgs,
+                                  const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
+                                  hipStream_t stream);
+// Work-groups of one shape that fit on a CU at this radius (LDS bound).
+int blocks_per_cu(int shape, int radius);
+
+// Adjoint (lpa_bwd.hip).  Tiles use shape 0 (32 x 32, 4 px per lane); the
+// tile queue and descriptors are laid out as for the batched forward.
+//   partial: [grid][num_weights + K] floats, one row per work-group of the
+//            first pass, written (not accumulated) by that pass
+//   dkernel: += reduction of the rows plus the variance term (may be null)
+hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDesc *d_imgs, const uint2 *d_tiles,
+                           int num_tiles, int *d_counter, int radius, int grid_prep, int grid_dx, float *partial,
+                           float *dkernel, hipStream_t stream);
+// Work-groups per CU of the two tile passes.
+void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, int *dx);
+
+}  // namespace lpa
+
+#endif

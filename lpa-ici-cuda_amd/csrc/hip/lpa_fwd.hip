@@ -1,0 +1,246 @@
+// Fused LPA-ICI forward for gfx950: one work-group owns an output tile,
+// stages the tile plus its halo in LDS once, and every lane walks the scales
+// of all directions for `px` adjacent pixels with the ICI intervals, the
+// selected estimate and the inverse-variance aggregation kept in registers.
+// Only the denoised image (and, optionally, a uint8 scale index per pixel and
+// direction) is written to HBM.
+//
+// Behaviour restated from the reference:
+//   tap order and zero padding   src/v0_0_lpa_ici_2D_naive.cpp:785-814
+//   ICI update, non-strict U>=L   src/v0_0_lpa_ici_2D_naive.cpp:889-903
+//   early exit on an empty intersection  src/v4_1_lpa_ici_2D_sparse_shared.cu:1077-1087
+//   aggregation order over d      src/v0_0_lpa_ici_2D_naive.cpp:917-927
+// The reference runs 16 walk launches + 16 accumulate launches + 1 normalize
+// launch per image and refills its LDS tile once per direction
+// (src/v4_1_lpa_ici_2D_sparse_shared.cu:1219-1235); here one launch covers
+// every direction and the tile is filled once.
+#include <hip/hip_runtime.h>
+
+#include "lpa_device.hpp"
+#include "lpa_tile.hpp"
+
+namespace lpa {
+namespace {
+
+// One output tile at (x0, y0) of an image.  `tile` is the work-group's LDS
+// buffer of lds_floats(TW, TH, R) floats.  Contains one barrier, reached by
+// every lane.
+template <int TW, int TH, int P, bool EXACT>
+__device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks,
+                                             const float *__restrict__ tab, const float *__restrict__ x,
+                                             float *__restrict__ y, uint8_t *__restrict__ map, int W, int H,
+                                             int pitch, int x0, int y0, int R)
+{
+    constexpr int LX = TW / P;
+    static_assert(LX * TH == kThreads, "tile shape must use 256 lanes");
+    const int lp = lds_pitch(TW, R);
+    const int tid = threadIdx.x;
+
+    // Stage tile + halo; pixels outside the image read as zero, which is the
+    // reference's zero padding.
+    stage_tile<TW, TH>(tile, lp, x, W, H, pitch, x0, y0, R);
+    __syncthreads();
+
+    const int tx = tid % LX, ty = tid / LX;
+    const int px0 = x0 + tx * P, py = y0 + ty;
+    const float *lane = tile + (ty + R) * lp + tx * P + R;
+    const int S = ks.S, K = ks.D * ks.S;
+    const float *__restrict__ thr_t = tab + K;
+    const float *__restrict__ inv_t = tab + 2 * K;
+
+    bool inside[P];
+    float num[P], den[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        inside[p] = py < H && px0 + p < W;
+        num[p] = 0.0f;
+        den[p] = 0.0f;
+    }
+    const size_t plane = (size_t)W * H;
+    const size_t pix = (size_t)py * W + px0;
+
+    for (int d = 0; d < ks.D; ++d) {
+        float lo[P], hi[P], best[P], binv[P];
+        int sel[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            // Pixels outside the image start with an empty interval.
+            lo[p] = inside[p] ? -INFINITY : INFINITY;
+            hi[p] = inside[p] ? INFINITY : -INFINITY;
+            best[p] = 0.0f;
+            binv[p] = 0.0f;
+            sel[p] = 0;
+        }
+        for (int s = 0; s < S; ++s) {
+            bool alive = false;
+#pragma unroll
+            for (int p = 0; p < P; ++p) alive |= hi[p] >= lo[p];
+            // The wave stops walking this direction once no lane has a
+            // non-empty interval left (an empty intersection stays empty).
+            if (!__any(alive)) break;
+            const int k = d * S + s;
+            float acc[P];
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[p] = 0.0f;
+            if (alive) correlate<P, EXACT>(lane, lp, ks, k, acc);
+            const float thr = thr_t[k], inv = inv_t[k];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float l = fmaxf(lo[p], acc[p] - thr);
+                const float h = fminf(hi[p], acc[p] + thr);
+                lo[p] = l;
+                hi[p] = h;
+                if (h >= l) {
+                    best[p] = acc[p];
+                    binv[p] = inv;
+                    sel[p] = s;
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            num[p] = mul_add_rounded(num[p], best[p], binv[p]);
+            den[p] = den[p] + binv[p];
+        }
+        if (map) {
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+                if (inside[p]) map[(size_t)d * plane + pix + p] = (uint8_t)sel[p];
+        }
+    }
+    float *__restrict__ yrow = y + (size_t)py * pitch + px0;
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+        if (inside[p]) yrow[p] = num[p] / den[p];
+}
+
+// Plain form: one launch per image.  Work-groups are dealt to the 8 XCDs in
+// turn, so block b runs tile (b % 8) * per_xcd + b / 8: consecutive tiles of
+// one XCD are neighbours and share their halos in that XCD's L2.
+template <int SH, bool EXACT>
+__global__ __launch_bounds__(kThreads) void fwd_plain_kernel(DevKset ks, ImgDesc img, int R, int num_tiles,
+                                                             int per_xcd)
+{
+    extern __shared__ float tile[];
+    constexpr TileShape sh = kShapes[SH];
+    const int b = blockIdx.x;
+    const int t = (b & 7) * per_xcd + (b >> 3);
+    if (t >= num_tiles) return;  // whole work-group, before any barrier
+    const int tx = t % img.tiles_x, ty = t / img.tiles_x;
+    process_tile<sh.tw, sh.th, sh.px, EXACT>(tile, ks, img.tab, img.x, img.y, img.map, img.W, img.H, img.pitch,
+                                             tx * sh.tw, ty * sh.th, R);
+}
+
+// Batched form: a persistent grid drains one tile queue that spans every
+// image of the batch.  tiles[t] = {image, (tile_y << 16) | tile_x}.  Every
+// work-group leaves the loop once the counter passes num_tiles.
+template <int SH, bool EXACT>
+__global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, const ImgDesc *__restrict__ imgs,
+                                                               const uint2 *__restrict__ tiles, int num_tiles,
+                                                               int *__restrict__ counter, int R)
+{
+    extern __shared__ float tile[];
+    __shared__ int next;
+    constexpr TileShape sh = kShapes[SH];
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = next;
+        if (t >= num_tiles) break;
+        const uint2 e = tiles[t];
+        const ImgDesc img = imgs[e.x];
+        process_tile<sh.tw, sh.th, sh.px, EXACT>(tile, ks, img.tab, img.x, img.y, img.map, img.W, img.H,
+                                                 img.pitch, (int)(e.y & 0xffffu) * sh.tw, (int)(e.y >> 16) * sh.th,
+                                                 R);
+        __syncthreads();  // the LDS tile and `next` are reused
+    }
+}
+
+__global__ void reset_counter_kernel(int *counter) { *counter = 0; }
+
+template <int SH>
+hipError_t plain_dispatch(bool exact, const DevKset &ks, const ImgDesc &img, int R, hipStream_t stream)
+{
+    constexpr TileShape sh = kShapes[SH];
+    const int tiles_y = (img.H + sh.th - 1) / sh.th;
+    const int num_tiles = img.tiles_x * tiles_y;
+    const int per_xcd = (num_tiles + 7) / 8;
+    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R);
+    if (exact)
+        hipLaunchKernelGGL((fwd_plain_kernel<SH, true>), dim3(8 * per_xcd), dim3(kThreads), lds, stream, ks, img, R,
+                           num_tiles, per_xcd);
+    else
+        hipLaunchKernelGGL((fwd_plain_kernel<SH, false>), dim3(8 * per_xcd), dim3(kThreads), lds, stream, ks, img,
+                           R, num_tiles, per_xcd);
+    return hipGetLastError();
+}
+
+template <int SH>
+hipError_t batched_dispatch(bool exact, const DevKset &ks, const ImgDesc *imgs, const uint2 *tiles, int num_tiles,
+                            int *counter, int R, int grid, hipStream_t stream)
+{
+    constexpr TileShape sh = kShapes[SH];
+    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+    if (exact)
+        hipLaunchKernelGGL((fwd_batched_kernel<SH, true>), dim3(grid), dim3(kThreads), lds, stream, ks, imgs, tiles,
+                           num_tiles, counter, R);
+    else
+        hipLaunchKernelGGL((fwd_batched_kernel<SH, false>), dim3(grid), dim3(kThreads), lds, stream, ks, imgs,
+                           tiles, num_tiles, counter, R);
+    return hipGetLastError();
+}
+
+template <int SH>
+int occupancy(int R)
+{
+    const size_t lds = sizeof(float) * (size_t)lds_floats(kShapes[SH].tw, kShapes[SH].th, R);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, false>, kThreads, lds) !=
+        hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
+}
+
+}  // namespace
+
+hipError_t launch_forward_plain(int shape, bool exact, const DevKset &ks, const ImgDesc &img, int radius,
+                                hipStream_t stream)
+{
+    switch (shape) {
+    case 0: return plain_dispatch<0>(exact, ks, img, radius, stream);
+    case 1: return plain_dispatch<1>(exact, ks, img, radius, stream);
+    case 2: return plain_dispatch<2>(exact, ks, img, radius, stream);
+    case 3: return plain_dispatch<3>(exact, ks, img, radius, stream);
+    case 4: return plain_dispatch<4>(exact, ks, img, radius, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_forward_batched(int shape, bool exact, const DevKset &ks, const ImgDesc *d_imgs,
+                                  const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
+                                  hipStream_t stream)
+{
+    switch (shape) {
+    case 0: return batched_dispatch<0>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 1: return batched_dispatch<1>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 2: return batched_dispatch<2>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 3: return batched_dispatch<3>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 4: return batched_dispatch<4>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int blocks_per_cu(int shape, int radius)
+{
+    switch (shape) {
+    case 0: return occupancy<0>(radius);
+    case 1: return occupancy<1>(radius);
+    case 2: return occupancy<2>(radius);
+    case 3: return occupancy<3>(radius);
+    case 4: return occupancy<4>(radius);
+    default: return 1;
+    }
+}
+
+}  // namespace lpa

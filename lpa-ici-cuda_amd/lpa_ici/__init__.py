@@ -1,0 +1,279 @@
+"""Python face of the MI355X LPA-ICI denoiser.
+
+Host helpers (kernel files, noise, PSNR, synthetic images, image I/O) take
+and return numpy arrays; device calls take torch tensors on the GPU and pass
+their pointers and the current HIP stream to the C ABI.  Every numeric path
+runs in liblpa_ici.so; nothing here computes a result itself.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (DEFAULT_KERNELS, FWD_EXACT, FWD_GRAPH, FWD_PLAIN, GAUSSIAN_M20_KERNELS, KERNEL_DIR, LIB_PATH,
+                   LpaError, LpaImage, check, lib)
+
+__all__ = [
+    "DEFAULT_KERNELS", "GAUSSIAN_M20_KERNELS", "KERNEL_DIR", "LIB_PATH", "FWD_PLAIN", "FWD_EXACT", "FWD_GRAPH",
+    "LpaError", "KernelSet", "Plan", "load_kernels", "kernel_variances", "make_noisy", "psnr", "synth_image",
+    "load_image", "write_hdr", "denoise", "backward", "image_struct", "tile_configs", "lib",
+]
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class KernelSet:
+    """A kernel set: the dense cropped kernels on the host (as the reference's
+    loader leaves them) and, once `.handle` is first used, the device-resident
+    run-length layout."""
+
+    def __init__(self, num_dirs, num_scales, weights, offsets, rows, cols, path=None):
+        self.D, self.S = int(num_dirs), int(num_scales)
+        self.weights, self.offsets = _f32(weights), _i32(offsets)
+        self.rows, self.cols = _i32(rows), _i32(cols)
+        self.path = path
+        self._handle = None
+
+    @property
+    def K(self):
+        return self.D * self.S
+
+    def kernel(self, d, s):
+        k = d * self.S + s
+        o, r, c = self.offsets[k], self.rows[k], self.cols[k]
+        return self.weights[o:o + r * c].reshape(r, c)
+
+    def variances(self, sigma):
+        return kernel_variances(self, sigma)
+
+    @property
+    def handle(self):
+        if self._handle is None:
+            h = ctypes.c_void_p()
+            check(lib().lpa_kset_create(self.D, self.S, _ptr(self.weights), _ptr(self.offsets), _ptr(self.rows),
+                                        _ptr(self.cols), ctypes.byref(h)))
+            self._handle = h
+        return self._handle
+
+    def info(self):
+        i = _lib.KsetInfo()
+        check(lib().lpa_kset_get_info(self.handle, ctypes.byref(i)))
+        return {n: getattr(i, n) for n, _ in _lib.KsetInfo._fields_}
+
+    def runs(self):
+        """Host copy of the run-length layout as numpy arrays."""
+        info = self.info()
+        R, NW, K = info["num_runs"], info["num_weights"], self.K
+        out = {n: np.zeros(R, np.int32) for n in ("run_row", "run_col", "run_len", "run_woff")}
+        out.update({n: np.zeros(K, np.int32) for n in ("kernel_first_run", "kernel_num_runs", "nnz")})
+        out["weights"] = np.zeros(max(NW, 1), np.float32)
+        out["sumsq"] = np.zeros(K, np.float32)
+        order = ["run_row", "run_col", "run_len", "run_woff", "kernel_first_run", "kernel_num_runs", "weights",
+                 "nnz", "sumsq"]
+        check(lib().lpa_kset_get_runs(self.handle, *[_ptr(out[n]) for n in order]))
+        out["weights"] = out["weights"][:NW]
+        return out
+
+    def set_tile_config(self, index):
+        check(lib().lpa_kset_set_tile_config(self.handle, int(index)))
+
+    def tile_config(self):
+        return lib().lpa_kset_get_tile_config(self.handle)
+
+    def close(self):
+        if self._handle is not None:
+            lib().lpa_kset_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_kernels(path=DEFAULT_KERNELS):
+    """load_LPA_kernels_standardized semantics (see include/lpa_ici.h)."""
+    D, S, nw = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    p = path.encode()
+    check(lib().lpa_load_kernels(p, ctypes.byref(D), ctypes.byref(S), ctypes.byref(nw), None, None, None, None))
+    w = np.zeros(nw.value, np.float32)
+    K = D.value * S.value
+    off, rows, cols = (np.zeros(K, np.int32) for _ in range(3))
+    check(lib().lpa_load_kernels(p, ctypes.byref(D), ctypes.byref(S), ctypes.byref(nw), _ptr(w), _ptr(off),
+                                 _ptr(rows), _ptr(cols)))
+    return KernelSet(D.value, S.value, w, off, rows, cols, path=path)
+
+
+def kernel_variances(ks, sigma):
+    out = np.zeros(ks.K, np.float32)
+    lib().lpa_kernel_variances(ks.K, _ptr(ks.weights), _ptr(ks.offsets), _ptr(ks.rows), _ptr(ks.cols),
+                               float(sigma), _ptr(out))
+    return out
+
+
+def make_noisy(clean, sigma, seed):
+    clean = _f32(clean)
+    out = np.empty_like(clean)
+    lib().lpa_make_noisy(_ptr(clean), _ptr(out), clean.size, float(sigma), int(seed) & 0xFFFFFFFF)
+    return out
+
+
+def psnr(a, b):
+    a, b = _f32(a), _f32(b)
+    assert a.size == b.size
+    return lib().lpa_psnr(_ptr(a), _ptr(b), a.size)
+
+
+def synth_image(width, height, seed):
+    out = np.empty((height, width), np.float32)
+    lib().lpa_synth_image(_ptr(out), int(width), int(height), int(seed) & 0xFFFFFFFF)
+    return out
+
+
+def load_image(path):
+    W, H = ctypes.c_int(), ctypes.c_int()
+    check(lib().lpa_load_image(path.encode(), ctypes.byref(W), ctypes.byref(H), None, 0))
+    out = np.empty((H.value, W.value), np.float32)
+    check(lib().lpa_load_image(path.encode(), ctypes.byref(W), ctypes.byref(H), _ptr(out), out.size))
+    return out
+
+
+def write_hdr(path, img):
+    img = _f32(img)
+    check(lib().lpa_write_hdr(path.encode(), img.shape[1], img.shape[0], _ptr(img)))
+
+
+def tile_configs():
+    out = []
+    for i in range(lib().lpa_num_tile_configs()):
+        c = _lib.TileConfig()
+        check(lib().lpa_tile_config_get(i, ctypes.byref(c)))
+        out.append({n: getattr(c, n) for n, _ in _lib.TileConfig._fields_})
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Device calls (torch tensors on the GPU)
+# ---------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def image_struct(x, y, scale_map=None, sigma=20.0 / 255.0, gamma=2.0, variances=None):
+    """An lpa_image over device tensors x, y (H x W float32, row stride =
+    pitch) and an optional uint8 [D, H, W] scale map."""
+    torch = _torch()
+    assert x.is_cuda and y.is_cuda and x.dtype == torch.float32 and y.dtype == torch.float32
+    assert x.dim() == 2 and x.shape == y.shape and x.stride(1) == 1 and y.stride(1) == 1
+    assert x.stride(0) == y.stride(0)
+    if scale_map is not None:
+        assert scale_map.dtype == torch.uint8 and scale_map.is_contiguous()
+        assert scale_map.shape[1:] == x.shape
+    img = LpaImage()
+    img.x, img.y = x.data_ptr(), y.data_ptr()
+    img.scale_map = scale_map.data_ptr() if scale_map is not None else None
+    img.height, img.width = int(x.shape[0]), int(x.shape[1])
+    img.pitch = int(x.stride(0))
+    img.sigma, img.gamma = float(sigma), float(gamma)
+    img.variances = variances.ctypes.data if variances is not None else None
+    return img
+
+
+def denoise(ks, xs, sigmas=20.0 / 255.0, gammas=2.0, scale_maps=False, flags=0, stream=None, variances=None):
+    """Denoise one (H x W tensor) or several (list) device images.
+
+    Returns y (and the uint8 [D,H,W] scale maps if scale_maps=True), a list
+    when a list went in.  flags: FWD_PLAIN for one launch per image, FWD_EXACT
+    for separately rounded mul/add (the CPU path's bits)."""
+    torch = _torch()
+    single = not isinstance(xs, (list, tuple))
+    xs = [xs] if single else list(xs)
+    n = len(xs)
+    sig = sigmas if isinstance(sigmas, (list, tuple)) else [sigmas] * n
+    gam = gammas if isinstance(gammas, (list, tuple)) else [gammas] * n
+    var = variances if isinstance(variances, (list, tuple)) else [variances] * n
+    var = [None if v is None else _f32(v) for v in var]
+    ys = [torch.empty_like(x) for x in xs]
+    maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device=x.device) for x in xs] \
+        if scale_maps else [None] * n
+    arr = (LpaImage * n)(*[image_struct(x, y, m, s, g, v) for x, y, m, s, g, v in zip(xs, ys, maps, sig, gam, var)])
+    check(lib().lpa_ici_forward(ks.handle, arr, n, int(flags), _stream_ptr(stream)))
+    out = (ys, maps) if scale_maps else ys
+    if single:
+        return (ys[0], maps[0]) if scale_maps else ys[0]
+    return out
+
+
+def backward(ks, xs, ys, maps, dys, sigmas=20.0 / 255.0, gammas=2.0, dkernel=None, stream=None):
+    """Adjoint of the forward with the scales frozen at `maps`.
+
+    xs: the forward's inputs, ys: its outputs, maps: its uint8 scale maps,
+    dys: upstream gradients (all lists of device tensors).  Returns (dxs,
+    dkernel) where dkernel (float32, one entry per run weight of the kernel
+    set, see KernelSet.runs()) is accumulated into if given."""
+    torch = _torch()
+    n = len(xs)
+    sig = sigmas if isinstance(sigmas, (list, tuple)) else [sigmas] * n
+    gam = gammas if isinstance(gammas, (list, tuple)) else [gammas] * n
+    dxs = [torch.empty_like(x) for x in xs]
+    for d, y in zip(dys, ys):
+        assert d.shape == y.shape and d.stride() == y.stride() and d.dtype == torch.float32
+    if dkernel is None:
+        dkernel = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device=xs[0].device)
+    arr = (LpaImage * n)(*[image_struct(x, y, m, s, g) for x, y, m, s, g in zip(xs, ys, maps, sig, gam)])
+    dy_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dys])
+    dx_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dxs])
+    check(lib().lpa_ici_backward(ks.handle, arr, dy_p, dx_p, ctypes.c_void_p(dkernel.data_ptr()), n,
+                                 _stream_ptr(stream)))
+    return dxs, dkernel
+
+
+class Plan:
+    """A fixed batch with its device state uploaded once; run() enqueues the
+    forward (a persistent launch, or one launch per image with FWD_PLAIN;
+    replayed from a hipGraph with FWD_GRAPH).  Keeps references to the
+    tensors it was built over."""
+
+    def __init__(self, ks, xs, ys, maps=None, sigmas=20.0 / 255.0, gammas=2.0, flags=0):
+        n = len(xs)
+        maps = maps if maps is not None else [None] * n
+        sig = sigmas if isinstance(sigmas, (list, tuple)) else [sigmas] * n
+        gam = gammas if isinstance(gammas, (list, tuple)) else [gammas] * n
+        self._keep = (ks, list(xs), list(ys), list(maps))
+        arr = (LpaImage * n)(*[image_struct(x, y, m, s, g) for x, y, m, s, g in zip(xs, ys, maps, sig, gam)])
+        h = ctypes.c_void_p()
+        check(lib().lpa_plan_create(ks.handle, arr, n, int(flags), ctypes.byref(h)))
+        self._h = h
+
+    def run(self, stream=None):
+        check(lib().lpa_plan_run(self._h, _stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            lib().lpa_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,203 @@
+"""Parity of the HIP forward with the oracle (CPU restatement of the
+reference's v0_0 path) and with fixtures made by the reference itself.
+
+Bars (the reference's algorithm in FP32 on the same seeded noisy image):
+  * exact mode (separately rounded mul/add, the reference's tap order):
+    bit-identical output and scale maps;
+  * fast mode (FMA): per-direction scale agreement >= 99.9 %, every
+    disagreement at an interval margin below 1e-6, |dy| <= 1e-5 wherever all
+    directions agree, |dPSNR| <= 0.005 dB; the error against the FP64 oracle
+    is printed for every case.
+"""
+import numpy as np
+import pytest
+import torch
+
+import lpa_ici
+import oracle
+from conftest import (GAMMA, GAUSSIAN_M20, SEED, SIGMA, SYMMETRIC, golden, kernels, noisy_synth)
+
+pytestmark = pytest.mark.gpu
+
+TOL_Y = 1e-5
+TOL_MARGIN = 1e-6
+MIN_AGREE = 0.999
+TOL_PSNR = 0.005
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def run(ks, x, flags=0, maps=True, sigma=SIGMA, gamma=GAMMA, variances=None):
+    out = lpa_ici.denoise(ks, dev(x), sigmas=sigma, gammas=gamma, scale_maps=maps, flags=flags, variances=variances)
+    torch.cuda.synchronize()
+    if maps:
+        return out[0].cpu().numpy(), out[1].cpu().numpy()
+    return out.cpu().numpy()
+
+
+def check_fast_parity(ks, x, clean, y, maps, label, sigma=SIGMA, gamma=GAMMA):
+    var = ks.variances(sigma)
+    yo, mo, marg = oracle.walk_f32(x, ks, var, gamma)
+    agree = maps == mo
+    per_dir = agree.reshape(ks.D, -1).mean(axis=1)
+    assert per_dir.min() >= MIN_AGREE, f"{label}: scale agreement {per_dir.min():.6f}"
+    bad = ~agree
+    assert np.all(marg[bad] < TOL_MARGIN), f"{label}: disagreement at margin {marg[bad].max()}"
+    all_agree = agree.all(axis=0)
+    dy = np.abs(y.astype(np.float64) - yo)
+    assert dy[all_agree].max(initial=0.0) <= TOL_Y, f"{label}: max |dy| {dy[all_agree].max()}"
+    y64, _ = oracle.walk_f64(x, ks, sigma, gamma)
+    e64 = np.abs(y - y64).max()
+    line = f"{label}: agree={per_dir.min():.6f} flips={int(bad.sum())} max|dy|={dy[all_agree].max():.2e} " \
+           f"max|y-y64|={e64:.2e}"
+    if clean is not None:
+        dp = abs(lpa_ici.psnr(clean, y) - lpa_ici.psnr(clean, yo))
+        assert dp <= TOL_PSNR, f"{label}: dPSNR {dp}"
+        line += f" dPSNR={dp:.2e}"
+    print(line)
+
+
+@pytest.mark.parametrize("tag,name", [("symmetric", SYMMETRIC), ("m20_gaussian", GAUSSIAN_M20),
+                                      ("d4_square", "lpa_kernels_m_1_0_d_4_square.txt")])
+def test_exact_matches_reference_fixture(tag, name):
+    """Exact mode reproduces the reference's v0_0 output bit for bit."""
+    ks = kernels(name)
+    x = golden("synth64_noisy.npy")
+    y = run(ks, x, flags=lpa_ici.FWD_EXACT, maps=False)
+    ref = golden(f"synth64_v0_0_{tag}.npy")
+    assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+
+
+def test_exact_cameraman_bitexact(ks):
+    x = golden("cameraman_noisy.npy")
+    y, maps = run(ks, x, flags=lpa_ici.FWD_EXACT)
+    assert np.array_equal(y.view(np.uint32), golden("cameraman_v0_0.npy").view(np.uint32))
+    _, mo, _ = oracle.walk_f32(x, ks, ks.variances(SIGMA), GAMMA)
+    assert np.array_equal(maps, mo)
+
+
+def test_fast_parity_config1_512(ks):
+    """Config 1: one 512x512 synthetic image, sigma 20/255, gamma 2."""
+    clean = lpa_ici.synth_image(512, 512, 1)
+    x = lpa_ici.make_noisy(clean, SIGMA, SEED)
+    y, maps = run(ks, x)
+    check_fast_parity(ks, x, clean, y, maps, "config1 512^2 symmetric")
+
+
+def test_fast_parity_gaussian_m20(ks):
+    ksg = kernels(GAUSSIAN_M20)
+    clean = lpa_ici.synth_image(256, 192, 3)
+    x = lpa_ici.make_noisy(clean, SIGMA, SEED)
+    y, maps = run(ksg, x)
+    check_fast_parity(ksg, x, clean, y, maps, "256x192 m20 gaussian")
+
+
+def test_fast_parity_barbara(ks):
+    """The reference's published case: PSNR 25.0736 on barbara."""
+    clean = lpa_ici.load_image(lpa_ici.KERNEL_DIR + "/../../tests/golden/images/barbara.png")
+    x = lpa_ici.make_noisy(clean, SIGMA, SEED)
+    y, maps = run(ks, x)
+    check_fast_parity(ks, x, clean, y, maps, "barbara")
+    assert round(lpa_ici.psnr(clean, y), 4) == 25.0736
+
+
+@pytest.mark.parametrize("shape", range(5))
+def test_tile_shapes_identical(ks, shape):
+    """Every tile shape adds each pixel's taps in the same order: same bits."""
+    x = noisy_synth(200, 136, seed=5)
+    base = run(ks, x, flags=lpa_ici.FWD_EXACT, maps=False)
+    ks.set_tile_config(shape)
+    try:
+        y = run(ks, x, flags=lpa_ici.FWD_EXACT, maps=False)
+        yf = run(ks, x, maps=False)
+    finally:
+        ks.set_tile_config(-1)
+    assert np.array_equal(y, base)
+    assert np.array_equal(yf, run(ks, x, maps=False))
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 9), (9, 1), (3, 3), (33, 65), (100, 3), (31, 31), (129, 47)])
+def test_small_and_ragged_exact(ks, w, h):
+    x = noisy_synth(w, h, seed=w * 100 + h)
+    y, maps = run(ks, x, flags=lpa_ici.FWD_EXACT)
+    yo, mo, _ = oracle.walk_f32(x, ks, ks.variances(SIGMA), GAMMA)
+    assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
+    assert np.array_equal(maps, mo)
+
+
+def test_pitched_image(ks):
+    """x and y with a row stride larger than the width."""
+    x = noisy_synth(70, 50, seed=11)
+    big = torch.zeros((50, 96), dtype=torch.float32, device="cuda")
+    big[:, :70] = dev(x)
+    xv = big[:, :70]
+    yb = torch.full((50, 96), -7.0, dtype=torch.float32, device="cuda")
+    yv = yb[:, :70]
+    img = lpa_ici.image_struct(xv, yv, None, SIGMA, GAMMA)
+    import ctypes
+    arr = (lpa_ici.LpaImage * 1)(img)
+    lpa_ici.check(lpa_ici.lib().lpa_ici_forward(kernels().handle, arr, 1, lpa_ici.FWD_EXACT,
+                                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    torch.cuda.synchronize()
+    yo, _, _ = oracle.walk_f32(x, kernels(), kernels().variances(SIGMA), GAMMA)
+    out = yb.cpu().numpy()
+    assert np.array_equal(out[:, :70], yo)
+    assert np.all(out[:, 70:] == -7.0), "pixels past the width were written"
+
+
+def test_explicit_variances(ks):
+    """The reference's interface passes variances in: same result as sigma."""
+    x = noisy_synth(64, 40, seed=2)
+    a = run(ks, x, flags=lpa_ici.FWD_EXACT, maps=False)
+    b = run(ks, x, flags=lpa_ici.FWD_EXACT, maps=False, variances=ks.variances(SIGMA), sigma=0.0)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["lpa_kernels_python_16_tapered.txt", "lpa_kernels_m_1_0_d_8_h12_1_15_15_30.txt",
+                                  "lpa_kernels_m_1_0_d_16_h12_1_2_4_8_16.txt"])
+def test_other_kernel_sets_exact(name):
+    """Sets beyond the reference's constant-memory budget are accepted."""
+    ksx = kernels(name)
+    x = noisy_synth(96, 80, seed=4)
+    y, maps = run(ksx, x, flags=lpa_ici.FWD_EXACT)
+    yo, mo, _ = oracle.walk_f32(x, ksx, ksx.variances(SIGMA), GAMMA)
+    assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
+    assert np.array_equal(maps, mo)
+
+
+def test_batched_equals_plain(ks):
+    """One persistent launch over uneven images equals one launch per image."""
+    rng = np.random.default_rng(7)
+    shapes = [(int(rng.integers(8, 300)), int(rng.integers(8, 300))) for _ in range(12)]
+    sig = [float(rng.choice([10, 20, 30, 40])) / 255.0 for _ in shapes]
+    gam = [float(rng.choice([1.5, 2.0, 2.5])) for _ in shapes]
+    xs = [dev(noisy_synth(w, h, seed=i, sigma=s)) for i, ((w, h), s) in enumerate(zip(shapes, sig))]
+    ya, ma = lpa_ici.denoise(ks, xs, sig, gam, scale_maps=True)
+    yb, mb = lpa_ici.denoise(ks, xs, sig, gam, scale_maps=True, flags=lpa_ici.FWD_PLAIN)
+    torch.cuda.synchronize()
+    for a, b, c, d in zip(ya, yb, ma, mb):
+        assert torch.equal(a, b) and torch.equal(c, d)
+    # each image also matches the oracle at its own sigma and gamma
+    for x, y, m, s, g in zip(xs[:4], ya[:4], ma[:4], sig[:4], gam[:4]):
+        check_fast_parity(ks, x.cpu().numpy(), None, y.cpu().numpy(), m.cpu().numpy(), "batched", sigma=s, gamma=g)
+
+
+def test_graph_replay_equals_eager(ks):
+    xs = [dev(noisy_synth(w, h, seed=w)) for w, h in [(130, 70), (64, 200), (257, 33)]]
+    ys_e = lpa_ici.denoise(ks, xs)
+    ys = [torch.empty_like(x) for x in xs]
+    stream = torch.cuda.Stream()
+    plan = lpa_ici.Plan(ks, xs, ys, flags=lpa_ici.FWD_GRAPH)
+    for i in range(4):
+        for y in ys:
+            y.fill_(float("nan"))
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            plan.run(stream)
+        stream.synchronize()
+        for a, b in zip(ys, ys_e):
+            assert torch.equal(a, b), f"replay {i}"
+        _ = (xs[0] * 2).sum().item()  # unrelated eager work between replays
+    plan.close()
