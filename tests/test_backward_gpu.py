@@ -1,0 +1,170 @@
+"""The HIP adjoint (selection frozen at the forward's scale maps) against
+the FP64 CPU adjoint and FP64 central differences.
+
+Bar: gradients within 1e-3 relative of FP64 central differences (eps 1e-3)
+at perturbations that flip no selection of the FP32 walk; against the exact
+FP64 adjoint the FP32 kernels are held to 1e-4 of the gradient's scale.
+"""
+import numpy as np
+import pytest
+import torch
+
+import lpa_ici
+import oracle
+from conftest import GAMMA, SIGMA, kernels, noisy_synth
+from lpa_ici.autograd import lpa_ici as lpa_ici_fn, template_taps
+
+pytestmark = pytest.mark.gpu
+EPS = 1e-3
+TOL_FD = 1e-3
+
+
+def forward_gpu(ks, x, sigma=SIGMA):
+    xd = torch.from_numpy(x).cuda()
+    y, m = lpa_ici.denoise(ks, xd, sigma, GAMMA, scale_maps=True)
+    return xd, y, m
+
+
+def dense_index(ks, r, i):
+    """Dense weight index of run weight i."""
+    k = int(np.searchsorted(r["kernel_first_run"], np.searchsorted(r["run_woff"], i, side="right") - 1,
+                            side="right") - 1)
+    run = int(np.searchsorted(r["run_woff"], i, side="right") - 1)
+    R, C = ks.rows[k], ks.cols[k]
+    row = r["run_row"][run] + R // 2
+    col = r["run_col"][run] + C // 2 + (i - r["run_woff"][run])
+    return k, int(ks.offsets[k] + row * C + col)
+
+
+class Modified:
+    """Kernel arrays with one dense weight changed (for the oracle)."""
+
+    def __init__(self, ks, idx, delta):
+        self.D, self.S, self.offsets, self.rows, self.cols = ks.D, ks.S, ks.offsets, ks.rows, ks.cols
+        self.weights = ks.weights.copy()
+        self.weights[idx] = np.float32(self.weights[idx] + delta)
+        self.K = ks.K
+
+    def variances(self, sigma):
+        return lpa_ici.kernel_variances(self, sigma)
+
+
+def loss64(x, g, ks, maps, sigma=SIGMA):
+    y, _ = oracle.walk_f64(x, ks, sigma, GAMMA, frozen=maps)
+    return float((g.astype(np.float64) * y).sum())
+
+
+def test_adjoint_matches_fp64_adjoint(ks):
+    x = noisy_synth(72, 56, seed=31)
+    xd, y, m = forward_gpu(ks, x)
+    g = torch.randn_like(xd)
+    dxs, dk = lpa_ici.backward(ks, [xd], [y], [m], [g], SIGMA, GAMMA)
+    torch.cuda.synchronize()
+    dx64, dkd64 = oracle.adjoint_f64(x, g.cpu().numpy(), ks, SIGMA, m.cpu().numpy())
+    dx = dxs[0].cpu().numpy()
+    e_dx = np.abs(dx - dx64).max() / np.abs(dx64).max()
+    r = ks.runs()
+    idx = np.array([dense_index(ks, r, i)[1] for i in range(len(r["weights"]))])
+    dk64 = dkd64[idx]
+    e_dk = np.abs(dk.cpu().numpy() - dk64).max() / np.abs(dk64).max()
+    print(f"adjoint vs FP64: dx rel {e_dx:.2e}, dk rel {e_dk:.2e}")
+    assert e_dx < 1e-4 and e_dk < 1e-4
+
+
+def test_adjoint_central_differences_image(ks):
+    x = noisy_synth(48, 40, seed=5)
+    xd, y, m = forward_gpu(ks, x)
+    maps = m.cpu().numpy()
+    g = torch.randn_like(xd)
+    gh = g.cpu().numpy()
+    dx = lpa_ici.backward(ks, [xd], [y], [m], [g], SIGMA, GAMMA)[0][0].cpu().numpy()
+    var = ks.variances(SIGMA)
+    rng = np.random.default_rng(0)
+    checked = 0
+    for _ in range(64):
+        r, c = int(rng.integers(0, 40)), int(rng.integers(0, 48))
+        xp, xm = x.copy(), x.copy()
+        xp[r, c] += EPS
+        xm[r, c] -= EPS
+        # skip perturbations that flip a selection of the FP32 walk
+        if not (np.array_equal(oracle.walk_f32(xp, ks, var, GAMMA)[1], maps) and
+                np.array_equal(oracle.walk_f32(xm, ks, var, GAMMA)[1], maps)):
+            continue
+        # the float32 inputs are not exactly x +- EPS: divide by the true step
+        step = float(np.float64(xp[r, c]) - np.float64(xm[r, c]))
+        fd = (loss64(xp, gh, ks, maps) - loss64(xm, gh, ks, maps)) / step
+        assert abs(dx[r, c] - fd) <= TOL_FD * max(abs(fd), 1e-2), (r, c, dx[r, c], fd)
+        checked += 1
+    assert checked >= 32
+
+
+def test_adjoint_central_differences_taps(ks):
+    x = noisy_synth(40, 40, seed=6)
+    xd, y, m = forward_gpu(ks, x)
+    maps = m.cpu().numpy()
+    g = torch.randn_like(xd)
+    gh = g.cpu().numpy()
+    dk = lpa_ici.backward(ks, [xd], [y], [m], [g], SIGMA, GAMMA)[1].cpu().numpy()
+    r = ks.runs()
+    used = set()
+    for d in range(ks.D):
+        for s in np.unique(maps[d]):
+            used.add(d * ks.S + int(s))
+    rng = np.random.default_rng(1)
+    checked = 0
+    for i in rng.permutation(len(r["weights"])):
+        k, di = dense_index(ks, r, int(i))
+        if k not in used or ks.weights[di] == 0:
+            continue
+        kp, km = Modified(ks, di, EPS), Modified(ks, di, -EPS)
+        if not (np.array_equal(oracle.walk_f32(x, kp, kp.variances(SIGMA), GAMMA)[1], maps) and
+                np.array_equal(oracle.walk_f32(x, km, km.variances(SIGMA), GAMMA)[1], maps)):
+            continue
+        step = float(np.float64(kp.weights[di]) - np.float64(km.weights[di]))
+        fd = (loss64(x, gh, kp, maps) - loss64(x, gh, km, maps)) / step
+        assert abs(dk[i] - fd) <= TOL_FD * max(abs(fd), 1e-2), (int(i), k, dk[i], fd)
+        checked += 1
+        if checked == 64:
+            break
+    assert checked >= 16
+
+
+def test_adjoint_batched_uneven(ks):
+    """One call over uneven images equals per-image calls (dx bit-identical,
+    dkernel the sum up to float reassociation)."""
+    shapes = [(70, 33), (128, 96), (31, 150)]
+    sig = [10 / 255, 30 / 255, 20 / 255]
+    xs, ys, ms, gs = [], [], [], []
+    for (w, h), s in zip(shapes, sig):
+        xd, y, m = forward_gpu(ks, noisy_synth(w, h, seed=w, sigma=s), s)
+        xs.append(xd), ys.append(y), ms.append(m), gs.append(torch.randn_like(xd))
+    dxs, dk = lpa_ici.backward(ks, xs, ys, ms, gs, sig, GAMMA)
+    dk_sum = torch.zeros_like(dk)
+    for i in range(3):
+        dx1, _ = lpa_ici.backward(ks, [xs[i]], [ys[i]], [ms[i]], [gs[i]], sig[i], GAMMA, dkernel=dk_sum)
+        assert torch.equal(dx1[0], dxs[i])
+    torch.cuda.synchronize()
+    assert torch.allclose(dk, dk_sum, rtol=1e-4, atol=1e-4 * dk.abs().max().item())
+
+
+def test_autograd_harness(ks):
+    x = noisy_synth(40, 36, seed=9)
+    xd = torch.from_numpy(x).cuda().requires_grad_(True)
+    taps = template_taps(ks).requires_grad_(True)
+    y = lpa_ici_fn(xd, taps, ks, SIGMA, GAMMA)
+    y_ref = lpa_ici.denoise(ks, torch.from_numpy(x).cuda(), SIGMA, GAMMA)
+    assert torch.equal(y.detach(), y_ref)
+    g = torch.randn_like(y)
+    (y * g).sum().backward()
+    _, m = lpa_ici.denoise(ks, xd.detach(), SIGMA, GAMMA, scale_maps=True)
+    dxs, dk = lpa_ici.backward(ks, [xd.detach()], [y_ref], [m], [g], SIGMA, GAMMA)
+    assert torch.allclose(xd.grad, dxs[0], rtol=0, atol=0)
+    assert torch.allclose(taps.grad, dk, rtol=1e-5, atol=1e-6 * dk.abs().max().item())
+
+
+def test_backward_needs_scale_map(ks):
+    xd = torch.from_numpy(noisy_synth(16, 16)).cuda()
+    y = lpa_ici.denoise(ks, xd)
+    with pytest.raises(lpa_ici.LpaError) as e:
+        lpa_ici.backward(ks, [xd], [y], [None], [torch.ones_like(y)])
+    assert e.value.status == 1 and "scale map" in str(e.value)
