@@ -144,7 +144,7 @@ def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
             "fwd_bwd_mpx_s": npx / ms_fb / 1e3}
 
 
-def cpu_baseline(x_host, crop=256):
+def cpu_baseline(x_host, crop=640):
     """The reference's v1_0 CPU code (fastest of its two CPU versions) on a
     crop of the workload image, one core; our restatement if the reference
     build is not present."""
@@ -155,12 +155,20 @@ def cpu_baseline(x_host, crop=256):
     h, w = x_host.shape
     sample = np.ascontiguousarray(x_host[(h - crop) // 2:(h + crop) // 2, (w - crop) // 2:(w + crop) // 2])
     kind = "reference" if oracle.ref_available("v1_0") else "port"
-    t = time.time()
-    if kind == "reference":
-        oracle.ref_denoise(sample, ks, var, GAMMA, version="v1_0")
-    else:
-        oracle.v1_0(sample, ks, var, GAMMA)
-    sec = time.time() - t
+    # the reference's code prints progress lines on stdout: keep them off the JSON stream
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        t = time.time()
+        if kind == "reference":
+            oracle.ref_denoise(sample, ks, var, GAMMA, version="v1_0")
+        else:
+            oracle.v1_0(sample, ks, var, GAMMA)
+        sec = time.time() - t
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
     cpu = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -185,7 +193,17 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-fwdbwd", action="store_true")
     ap.add_argument("--tile-config", type=int, default=-1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (the reference kernel for the HBM byte counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the denoiser has no multi-GPU path in this build "
               "(images shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
