@@ -211,10 +211,18 @@ def main():
     torch.cuda.init()
     stream = torch.cuda.Stream()
     ks = lpa_ici.load_kernels()
-    ks.set_tile_config(args.tile_config)
     t0 = time.time()
     x_host = make_image(args.size, args.size, 42, SIGMA, SEED)
     log(f"[setup] {args.size}^2 image in {time.time() - t0:.1f} s; device {torch.cuda.get_device_name(0)}")
+    tune = {}
+    if args.tile_config >= 0:
+        ks.set_tile_config(args.tile_config)
+    else:
+        from lpa_ici.autotune import autotune
+        best, times = autotune(ks, [torch.from_numpy(x_host).cuda()], flags=lpa_ici.FWD_PLAIN, stream=stream,
+                               cache=None)
+        tune = {str(k): round(v, 4) for k, v in times.items()}
+        log(f"[autotune] plain 4096^2: best config {best}; ms per config {tune}")
 
     main_leg = plain_leg(ks, x_host, args.steps, args.warmup, stream, "plain symmetric")
     npx = args.size * args.size
@@ -233,7 +241,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain {args.size}x{args.size} fp32 synthetic (polygons, disks, ramps, stripes), "
                                "sigma 20/255, gamma 2, 16x7 symmetric m=(1,0) set, one launch per pass",
-                   "pixels": npx, "tile_config": ks.tile_config()},
+                   "pixels": npx, "tile_config": ks.tile_config(), "autotune_ms": tune,
+                   "tile_configs": lpa_ici.tile_configs()},
         "roofline": {"bound": "valu", "achieved": round(main_leg["tflops"], 3), "peak": MI355X_FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(main_leg["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4),
                      "traffic": None,
@@ -245,7 +254,7 @@ def main():
         log(f"[cpu] {rec['cpu_baseline']}")
     if not args.skip_gaussian:
         ksg = lpa_ici.load_kernels(lpa_ici.GAUSSIAN_M20_KERNELS)
-        ksg.set_tile_config(args.tile_config)
+        ksg.set_tile_config(ks.tile_config())
         g = plain_leg(ksg, x_host, args.steps, args.warmup, stream, "plain m20 gaussian")
         rec["gaussian_m20"] = {"mpx_s": round(g["mpx_s"], 2), "ms": round(g["ms"], 4),
                                "tflops": round(g["tflops"], 3), "taps_per_px": round(g["taps_per_px"], 1)}

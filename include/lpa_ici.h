@@ -133,6 +133,7 @@ void lpa_plan_destroy(lpa_plan_t plan);
 typedef struct lpa_tile_config {
     int tile_w, tile_h;   /* output pixels per work-group tile */
     int px_per_lane;      /* adjacent output pixels one lane computes */
+    int body;             /* inner loop: 1 = one unrolled body per run length, 0 = sliding-window loop */
     int halo;             /* LDS halo radius, >= the kernel set's max radius */
     int blocks_per_cu;    /* persistent work-groups per CU (batched form) */
 } lpa_tile_config;

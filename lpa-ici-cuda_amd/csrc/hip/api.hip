@@ -12,6 +12,7 @@
 #include "../internal.hpp"
 #include "lpa_device.hpp"
 #include "lpa_ici.h"
+#include "lpa_tile.hpp"
 
 using lpa::fail;
 
@@ -97,10 +98,14 @@ void pack_runs(lpa_kset *ks, const float *dense, const int *offsets, const int *
                     last = c;
                 }
             if (first < 0) continue;
-            const int dr = r - R / 2, dc = first - C / 2, len = last - first + 1;
-            ks->runs.push_back(int4{dr, dc, len, (int)ks->weights.size()});
-            for (int c = first; c <= last; ++c) ks->weights.push_back(w[r * C + c]);
-            radius = std::max(radius, std::max(std::abs(dr), std::max(std::abs(dc), std::abs(dc + len - 1))));
+            // A run longer than the kernels' longest unrolled body is cut into
+            // consecutive pieces; the per-pixel tap order is unchanged.
+            for (int c0 = first; c0 <= last; c0 += lpa::kMaxRun) {
+                const int dr = r - R / 2, dc = c0 - C / 2, len = std::min(lpa::kMaxRun, last - c0 + 1);
+                ks->runs.push_back(int4{dr, dc, len, (int)ks->weights.size()});
+                for (int c = c0; c < c0 + len; ++c) ks->weights.push_back(w[r * C + c]);
+                radius = std::max(radius, std::max(std::abs(dr), std::max(std::abs(dc), std::abs(dc + len - 1))));
+            }
         }
         ks->kern.push_back(int2{first_run, (int)ks->runs.size() - first_run});
         const int first_w = first_run < (int)ks->runs.size() ? ks->runs[first_run].w : (int)ks->weights.size();
@@ -395,6 +400,7 @@ lpa_status lpa_tile_config_get(int index, lpa_tile_config *cfg)
     cfg->tile_w = lpa::kShapes[index].tw;
     cfg->tile_h = lpa::kShapes[index].th;
     cfg->px_per_lane = lpa::kShapes[index].px;
+    cfg->body = lpa::kShapes[index].body;
     cfg->halo = lpa::kMaxRadius;
     cfg->blocks_per_cu = 0;
     return LPA_OK;

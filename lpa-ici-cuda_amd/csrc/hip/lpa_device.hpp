@@ -33,18 +33,22 @@ struct ImgDesc {
 };
 
 // Compiled tile shapes: output tile tw x th pixels, px adjacent pixels per
-// lane along a row, 256 lanes per work-group.
+// lane along a row, 256 lanes per work-group, and the inner-loop body
+// (1: length-dispatched unrolled runs, 0: sliding-window loop).
 struct TileShape {
-    int tw, th, px;
+    int tw, th, px, body;
 };
 constexpr int kThreads = 256;
-constexpr int kNumShapes = 5;
+constexpr int kNumShapes = 8;
 constexpr TileShape kShapes[kNumShapes] = {
-    {32, 32, 4},  // 0: wave covers 32x8 px
-    {64, 16, 4},  // 1: wave covers 64x4 px
-    {32, 16, 2},  // 2: wave covers 32x4 px
-    {16, 16, 1},  // 3: one pixel per lane, wave covers 16x4 px
-    {64, 32, 8},  // 4: wave covers 64x8 px
+    {32, 32, 4, 1},  // 0: wave covers 32x8 px
+    {32, 32, 4, 0},  // 1: same, window loop
+    {64, 16, 4, 1},  // 2: wave covers 64x4 px
+    {32, 16, 2, 1},  // 3: wave covers 32x4 px
+    {16, 16, 1, 1},  // 4: one pixel per lane, wave covers 16x4 px
+    {64, 32, 8, 1},  // 5: wave covers 64x8 px
+    {64, 16, 4, 0},  // 6: wave covers 64x4 px, window loop
+    {32, 16, 2, 0},  // 7: wave covers 32x4 px, window loop
 };
 constexpr int kMaxRadius = 31;
 

@@ -25,7 +25,7 @@ namespace {
 // One output tile at (x0, y0) of an image.  `tile` is the work-group's LDS
 // buffer of lds_floats(TW, TH, R) floats.  Contains one barrier, reached by
 // every lane.
-template <int TW, int TH, int P, bool EXACT>
+template <int TW, int TH, int P, bool EXACT, int BODY>
 __device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks,
                                              const float *__restrict__ tab, const float *__restrict__ x,
                                              float *__restrict__ y, uint8_t *__restrict__ map, int W, int H,
@@ -82,7 +82,7 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
             float acc[P];
 #pragma unroll
             for (int p = 0; p < P; ++p) acc[p] = 0.0f;
-            if (alive) correlate<P, EXACT>(lane, lp, ks, k, acc);
+            if (alive) correlate<P, EXACT, BODY>(lane, lp, ks, k, acc);
             const float thr = thr_t[k], inv = inv_t[k];
 #pragma unroll
             for (int p = 0; p < P; ++p) {
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kThreads) void fwd_plain_kernel(DevKset ks, ImgDesc
     const int t = (b & 7) * per_xcd + (b >> 3);
     if (t >= num_tiles) return;  // whole work-group, before any barrier
     const int tx = t % img.tiles_x, ty = t / img.tiles_x;
-    process_tile<sh.tw, sh.th, sh.px, EXACT>(tile, ks, img.tab, img.x, img.y, img.map, img.W, img.H, img.pitch,
+    process_tile<sh.tw, sh.th, sh.px, EXACT, sh.body>(tile, ks, img.tab, img.x, img.y, img.map, img.W, img.H, img.pitch,
                                              tx * sh.tw, ty * sh.th, R);
 }
 
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, const
         if (t >= num_tiles) break;
         const uint2 e = tiles[t];
         const ImgDesc img = imgs[e.x];
-        process_tile<sh.tw, sh.th, sh.px, EXACT>(tile, ks, img.tab, img.x, img.y, img.map, img.W, img.H,
+        process_tile<sh.tw, sh.th, sh.px, EXACT, sh.body>(tile, ks, img.tab, img.x, img.y, img.map, img.W, img.H,
                                                  img.pitch, (int)(e.y & 0xffffu) * sh.tw, (int)(e.y >> 16) * sh.th,
                                                  R);
         __syncthreads();  // the LDS tile and `next` are reused
@@ -213,6 +213,9 @@ hipError_t launch_forward_plain(int shape, bool exact, const DevKset &ks, const 
     case 2: return plain_dispatch<2>(exact, ks, img, radius, stream);
     case 3: return plain_dispatch<3>(exact, ks, img, radius, stream);
     case 4: return plain_dispatch<4>(exact, ks, img, radius, stream);
+    case 5: return plain_dispatch<5>(exact, ks, img, radius, stream);
+    case 6: return plain_dispatch<6>(exact, ks, img, radius, stream);
+    case 7: return plain_dispatch<7>(exact, ks, img, radius, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -227,6 +230,9 @@ hipError_t launch_forward_batched(int shape, bool exact, const DevKset &ks, cons
     case 2: return batched_dispatch<2>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
     case 3: return batched_dispatch<3>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
     case 4: return batched_dispatch<4>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 5: return batched_dispatch<5>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 6: return batched_dispatch<6>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 7: return batched_dispatch<7>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -239,6 +245,9 @@ int blocks_per_cu(int shape, int radius)
     case 2: return occupancy<2>(radius);
     case 3: return occupancy<3>(radius);
     case 4: return occupancy<4>(radius);
+    case 5: return occupancy<5>(radius);
+    case 6: return occupancy<6>(radius);
+    case 7: return occupancy<7>(radius);
     default: return 1;
     }
 }

@@ -25,47 +25,95 @@ __device__ __forceinline__ float tap(float acc, float w, float v)
         return __builtin_fmaf(w, v, acc);
 }
 
+// One run of exactly L taps for the P pixels starting at `src`: L + P - 1
+// tile values and L weights, then L*P FMAs.  Per pixel the taps are added in
+// order along the run.
+template <int L, int P, bool EXACT>
+__device__ __forceinline__ void run_taps(const float *__restrict__ src, const float *__restrict__ w, float (&acc)[P])
+{
+    float v[L + P - 1];
+#pragma unroll
+    for (int q = 0; q < L + P - 1; ++q) v[q] = src[q];
+    float c[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) c[j] = w[j];
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] = tap<EXACT>(acc[p], c[j], v[j + p]);
+}
+
+// Sliding-window body: a window of P+3 tile values moves along the run four
+// taps at a time, then one tap at a time for the rest.  Smaller code than the
+// unrolled body; costs window moves and a loop tail.
+template <int P, bool EXACT>
+__device__ __forceinline__ void run_window(const float *__restrict__ src, const float *__restrict__ w, int len,
+                                           float (&acc)[P])
+{
+    float v[P + 3];
+#pragma unroll
+    for (int q = 0; q < P - 1; ++q) v[q] = src[q];
+    int j = 0;
+    for (; j + 4 <= len; j += 4) {
+#pragma unroll
+        for (int q = P - 1; q < P + 3; ++q) v[q] = src[j + q];
+        const float w0 = w[j], w1 = w[j + 1], w2 = w[j + 2], w3 = w[j + 3];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            acc[p] = tap<EXACT>(acc[p], w0, v[p]);
+            acc[p] = tap<EXACT>(acc[p], w1, v[p + 1]);
+            acc[p] = tap<EXACT>(acc[p], w2, v[p + 2]);
+            acc[p] = tap<EXACT>(acc[p], w3, v[p + 3]);
+        }
+#pragma unroll
+        for (int q = 0; q < P - 1; ++q) v[q] = v[q + 4];
+    }
+    for (; j < len; ++j) {
+        v[P - 1] = src[j + P - 1];
+        const float wj = w[j];
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] = tap<EXACT>(acc[p], wj, v[p]);
+#pragma unroll
+        for (int q = 0; q < P - 1; ++q) v[q] = v[q + 1];
+    }
+}
+
+// Runs are at most kMaxRun taps long (the packer splits longer ones).
+constexpr int kMaxRun = 32;
+
 // acc[p] += sum over kernel k's runs of w[j] * tile[row, col + p + j], for the
 // P pixels starting at `lane` (the LDS address of the lane's first pixel).
 // Per pixel the taps are added in run order, then in order along the run:
-// the row-major order of the dense kernel.  A window of P+3 tile values slides
-// along the run four taps at a time, so each LDS value read feeds up to P
-// FMAs; weights and run records are wave-uniform scalar loads.
-template <int P, bool EXACT>
+// the row-major order of the dense kernel.  Each LDS value read feeds up to P
+// FMAs; the run's address costs one add from the lane's base; weights and
+// run records are wave-uniform scalar loads.  BODY 1: the run length is the
+// same for every lane, so a switch picks one fully unrolled body per run (no
+// window shuffling, no loop tail).  BODY 0: the sliding-window loop.
+template <int P, bool EXACT, int BODY = 1>
 __device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp, const DevKset &ks, int k,
                                           float (&acc)[P])
 {
     const int2 kr = ks.kern[k];
     for (int r = 0; r < kr.y; ++r) {
         const int4 run = ks.runs[kr.x + r];
-        const float *__restrict__ src = lane + run.x * lp + run.y;
+        const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
+        const float *__restrict__ src = lane + off;
         const float *__restrict__ w = ks.weights + run.w;
-        const int len = run.z;
-        float v[P + 3];
-#pragma unroll
-        for (int q = 0; q < P - 1; ++q) v[q] = src[q];
-        int j = 0;
-        for (; j + 4 <= len; j += 4) {
-#pragma unroll
-            for (int q = P - 1; q < P + 3; ++q) v[q] = src[j + q];
-            const float w0 = w[j], w1 = w[j + 1], w2 = w[j + 2], w3 = w[j + 3];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                acc[p] = tap<EXACT>(acc[p], w0, v[p]);
-                acc[p] = tap<EXACT>(acc[p], w1, v[p + 1]);
-                acc[p] = tap<EXACT>(acc[p], w2, v[p + 2]);
-                acc[p] = tap<EXACT>(acc[p], w3, v[p + 3]);
-            }
-#pragma unroll
-            for (int q = 0; q < P - 1; ++q) v[q] = v[q + 4];
+        if constexpr (BODY == 0) {
+            run_window<P, EXACT>(src, w, run.z, acc);
+            continue;
         }
-        for (; j < len; ++j) {
-            v[P - 1] = src[j + P - 1];
-            const float wj = w[j];
-#pragma unroll
-            for (int p = 0; p < P; ++p) acc[p] = tap<EXACT>(acc[p], wj, v[p]);
-#pragma unroll
-            for (int q = 0; q < P - 1; ++q) v[q] = v[q + 1];
+        switch (run.z) {
+#define LPA_RUN_CASE(L)                                                                                            \
+    case L: run_taps<L, P, EXACT>(src, w, acc); break;
+            LPA_RUN_CASE(1) LPA_RUN_CASE(2) LPA_RUN_CASE(3) LPA_RUN_CASE(4) LPA_RUN_CASE(5) LPA_RUN_CASE(6)
+            LPA_RUN_CASE(7) LPA_RUN_CASE(8) LPA_RUN_CASE(9) LPA_RUN_CASE(10) LPA_RUN_CASE(11) LPA_RUN_CASE(12)
+            LPA_RUN_CASE(13) LPA_RUN_CASE(14) LPA_RUN_CASE(15) LPA_RUN_CASE(16) LPA_RUN_CASE(17) LPA_RUN_CASE(18)
+            LPA_RUN_CASE(19) LPA_RUN_CASE(20) LPA_RUN_CASE(21) LPA_RUN_CASE(22) LPA_RUN_CASE(23) LPA_RUN_CASE(24)
+            LPA_RUN_CASE(25) LPA_RUN_CASE(26) LPA_RUN_CASE(27) LPA_RUN_CASE(28) LPA_RUN_CASE(29) LPA_RUN_CASE(30)
+            LPA_RUN_CASE(31) LPA_RUN_CASE(32)
+#undef LPA_RUN_CASE
+        default: break;
         }
     }
 }

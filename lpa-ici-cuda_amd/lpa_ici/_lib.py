@@ -52,7 +52,7 @@ class KsetInfo(ctypes.Structure):
 
 
 class TileConfig(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int) for n in ("tile_w", "tile_h", "px_per_lane", "halo", "blocks_per_cu")]
+    _fields_ = [(n, ctypes.c_int) for n in ("tile_w", "tile_h", "px_per_lane", "body", "halo", "blocks_per_cu")]
 
 
 _P = ctypes.c_void_p

@@ -103,7 +103,7 @@ def test_fast_parity_barbara(ks):
     assert round(lpa_ici.psnr(clean, y), 4) == 25.0736
 
 
-@pytest.mark.parametrize("shape", range(5))
+@pytest.mark.parametrize("shape", range(len(lpa_ici.tile_configs())))
 def test_tile_shapes_identical(ks, shape):
     """Every tile shape adds each pixel's taps in the same order: same bits."""
     x = noisy_synth(200, 136, seed=5)
@@ -201,3 +201,16 @@ def test_graph_replay_equals_eager(ks):
             assert torch.equal(a, b), f"replay {i}"
         _ = (xs[0] * 2).sum().item()  # unrelated eager work between replays
     plan.close()
+
+
+def test_autotune_installs_fastest(ks, tmp_path):
+    from lpa_ici.autotune import autotune
+    xs = [dev(noisy_synth(256, 192, seed=3))]
+    best, times = autotune(ks, xs, cache=str(tmp_path / "cache.json"), reps=1)
+    try:
+        assert ks.tile_config() == best and times[best] == min(times.values())
+        assert len(times) == len(lpa_ici.tile_configs())
+        again, t2 = autotune(ks, xs, cache=str(tmp_path / "cache.json"))
+        assert again == best and t2 == {}  # served from the cache
+    finally:
+        ks.set_tile_config(-1)
