@@ -451,24 +451,29 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     hipStream_t s = (hipStream_t)stream;
     const int K = ks->K, NW = (int)ks->weights.size();
     // Tiles and tables as for the batched forward, shape 0.
-    std::vector<uint2> tiles;
-    for (int i = 0; i < n; ++i) {
-        const int tx = tiles_x_of(0, imgs[i].width), ty = tiles_y_of(0, imgs[i].height);
-        for (int y = 0; y < ty; ++y)
-            for (int x = 0; x < tx; ++x) tiles.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
-    }
+    std::vector<uint2> tiles, dx_tiles;
+    auto add_tiles = [&](std::vector<uint2> &v, int edge) {
+        for (int i = 0; i < n; ++i) {
+            const int tx = (imgs[i].width + edge - 1) / edge, ty = (imgs[i].height + edge - 1) / edge;
+            for (int y = 0; y < ty; ++y)
+                for (int x = 0; x < tx; ++x) v.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
+        }
+    };
+    add_tiles(tiles, lpa::kBwdTile);
+    add_tiles(dx_tiles, lpa::kDxTile);
     int bp = 1, bd = 1;
     lpa::backward_occupancy(ks->radius, ks->S, K, &bp, &bd);
     const int cus = num_cus();
     const int grid_prep = std::max(1, std::min((int)tiles.size(), cus * bp));
-    const int grid_dx = std::max(1, std::min((int)tiles.size(), cus * bd));
+    const int grid_dx = std::max(1, std::min((int)dx_tiles.size(), cus * bd));
     size_t qpx = 0;
     for (int i = 0; i < n; ++i) qpx += (size_t)imgs[i].width * imgs[i].height;
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t off_desc = 256;
     const size_t off_tab = off_desc + up(sizeof(lpa::BwdDesc) * n);
     const size_t off_tiles = off_tab + up(sizeof(float) * 3 * K * n);
-    const size_t off_q = off_tiles + up(sizeof(uint2) * tiles.size());
+    const size_t off_dxt = off_tiles + up(sizeof(uint2) * tiles.size());
+    const size_t off_q = off_dxt + up(sizeof(uint2) * dx_tiles.size());
     const size_t off_part = off_q + up(sizeof(float) * qpx);
     const size_t bytes = off_part + sizeof(float) * ((size_t)grid_prep * (NW + K) + K);
     const size_t host_bytes = off_q;
@@ -498,11 +503,13 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     }
     std::memcpy(host.data() + off_desc, desc.data(), sizeof(lpa::BwdDesc) * n);
     std::memcpy(host.data() + off_tiles, tiles.data(), sizeof(uint2) * tiles.size());
+    std::memcpy(host.data() + off_dxt, dx_tiles.data(), sizeof(uint2) * dx_tiles.size());
     hipError_t e = hipMemcpyAsync(mem, host.data(), host_bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
     if (e == hipSuccess)
         e = lpa::launch_backward(ks->dev(), ks->adjoint(), (const lpa::BwdDesc *)(base + off_desc),
-                                 (const uint2 *)(base + off_tiles), (int)tiles.size(), (int *)base, ks->radius,
+                                 (const uint2 *)(base + off_tiles), (int)tiles.size(),
+                                 (const uint2 *)(base + off_dxt), (int)dx_tiles.size(), (int *)base, ks->radius,
                                  grid_prep, grid_dx, (float *)(base + off_part), dkernel, s);
     hipError_t ef = hipFreeAsync(mem, s);
     if (e != hipSuccess) return hip_fail(e, "backward");

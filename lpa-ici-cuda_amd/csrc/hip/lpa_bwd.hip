@@ -245,26 +245,26 @@ __global__ __launch_bounds__(kThreads) void bwd_prep_kernel(DevKset ks, DevAdjoi
     for (int k = tid; k < K; k += kThreads) row[NW + k] = eblk[k];
 }
 
-// Pass 2.  Each lane keeps q and the scale index of up to 48 tile+halo
-// elements in registers (rows tid/64 + 4i, columns tid%64 + 64j); the LDS
-// holds only the masked map and the table.
-constexpr int kSlotRows = 24, kSlotCols = 2;
+// Pass 2 (dx), on 64 x 64 tiles with 1024 lanes (4 px per lane).  Each lane
+// keeps q and the scale index of 16 tile+halo elements in registers (rows
+// tid/64 + 16i, columns tid%64 + 64j); the LDS holds only the masked map and
+// the table.  The wide tile cuts the masked-map building per output pixel
+// (halo overhead (64+2R)^2/64^2 instead of (32+2R)^2/32^2).
+constexpr int DXW = 64, DXH = 64, DXP = 4, DXT = 1024, DXLX = DXW / DXP;
+constexpr int kSlotRows = 8, kSlotCols = 2;  // 16 + 16i rows, 64 + 64j columns cover 126 x 126
 
-__global__ __launch_bounds__(kThreads) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
-                                                          const BwdDesc *__restrict__ imgs,
-                                                          const uint2 *__restrict__ tiles, int num_tiles,
-                                                          int *__restrict__ counter, int R)
+__global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj, const BwdDesc *__restrict__ imgs,
+                                                     const uint2 *__restrict__ tiles, int num_tiles,
+                                                     int *__restrict__ counter, int R)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int next;
     const int S = ks.S, D = ks.D, K = D * S;
-    const int lp = lds_pitch(TW, R);
+    const int lp = lds_pitch(DXW, R);
     float *mt = (float *)lds;
-    float *tab = mt + lds_floats(TW, TH, R);
     const int tid = threadIdx.x;
-    const int fw = TW + 2 * R, fh = TH + 2 * R;
+    const int fw = DXW + 2 * R, fh = DXH + 2 * R;
     const int r0 = tid >> 6, c0 = tid & 63;
-    (void)adj;
 
     for (;;) {
         if (tid == 0) next = atomicAdd(counter + 1, 1);
@@ -274,9 +274,8 @@ __global__ __launch_bounds__(kThreads) void bwd_dx_kernel(DevKset ks, DevAdjoint
         const uint2 te = tiles[t];
         const BwdDesc bd = imgs[te.x];
         const ImgDesc &img = bd.img;
-        const int x0 = (int)(te.y & 0xffffu) * TW, y0 = (int)(te.y >> 16) * TH;
-        for (int i = tid; i < 3 * K; i += kThreads) tab[i] = img.tab[i];
-        const float *inv_t = tab + 2 * K;
+        const int x0 = (int)(te.y & 0xffffu) * DXW, y0 = (int)(te.y >> 16) * DXH;
+        const float *__restrict__ inv_t = img.tab + 2 * K;  // wave-uniform scalar loads
         const size_t plane = (size_t)img.W * img.H;
 
         float qv[kSlotRows][kSlotCols];
@@ -284,39 +283,39 @@ __global__ __launch_bounds__(kThreads) void bwd_dx_kernel(DevKset ks, DevAdjoint
         for (int i = 0; i < kSlotRows; ++i)
 #pragma unroll
             for (int j = 0; j < kSlotCols; ++j) {
-                const int r = r0 + 4 * i, c = c0 + 64 * j;
+                const int r = r0 + 16 * i, c = c0 + 64 * j;
                 const int gy = y0 - R + r, gx = x0 - R + c;
                 const bool in = r < fh && c < fw && gy >= 0 && gy < img.H && gx >= 0 && gx < img.W;
                 qv[i][j] = in ? bd.q[(size_t)gy * img.W + gx] : 0.0f;
             }
 
-        const int tx = tid % LX, ty = tid / LX;
-        const int px0 = x0 + tx * P, py = y0 + ty;
-        const float *lane_m = mt + (ty + R) * lp + tx * P + R;
-        float dx[P];
+        const int tx = tid % DXLX, ty = tid / DXLX;
+        const int px0 = x0 + tx * DXP, py = y0 + ty;
+        const float *lane_m = mt + (ty + R) * lp + tx * DXP + R;
+        float dx[DXP];
 #pragma unroll
-        for (int p = 0; p < P; ++p) dx[p] = 0.0f;
+        for (int p = 0; p < DXP; ++p) dx[p] = 0.0f;
 
         for (int d = 0; d < D; ++d) {
-            uint8_t sv[kSlotRows][kSlotCols];
+            int sv[kSlotRows][kSlotCols];
 #pragma unroll
             for (int i = 0; i < kSlotRows; ++i)
 #pragma unroll
                 for (int j = 0; j < kSlotCols; ++j) {
-                    const int r = r0 + 4 * i, c = c0 + 64 * j;
+                    const int r = r0 + 16 * i, c = c0 + 64 * j;
                     const int gy = y0 - R + r, gx = x0 - R + c;
                     const bool in = r < fh && c < fw && gy >= 0 && gy < img.H && gx >= 0 && gx < img.W;
                     sv[i][j] = in ? img.map[(size_t)d * plane + (size_t)gy * img.W + gx] : 255;
                 }
             for (int s = 0; s < S; ++s) {
                 int any = 0;
-                __syncthreads();  // previous correlation done with the map; table staged
+                __syncthreads();  // previous correlation done with the map
                 const float inv = inv_t[d * S + s];
 #pragma unroll
                 for (int i = 0; i < kSlotRows; ++i)
 #pragma unroll
                     for (int j = 0; j < kSlotCols; ++j) {
-                        const int r = r0 + 4 * i, c = c0 + 64 * j;
+                        const int r = r0 + 16 * i, c = c0 + 64 * j;
                         if (r < fh && c < fw) {
                             const bool m = sv[i][j] == s && qv[i][j] != 0.0f;
                             any |= m;
@@ -324,12 +323,12 @@ __global__ __launch_bounds__(kThreads) void bwd_dx_kernel(DevKset ks, DevAdjoint
                         }
                     }
                 if (!__syncthreads_or(any)) continue;
-                correlate<P, false>(lane_m, lp, adj.flip, d * S + s, dx);
+                correlate<DXP, false, 0>(lane_m, lp, adj.flip, d * S + s, dx);
             }
         }
         float *__restrict__ out = bd.dx + (size_t)py * img.pitch + px0;
 #pragma unroll
-        for (int p = 0; p < P; ++p)
+        for (int p = 0; p < DXP; ++p)
             if (py < img.H && px0 + p < img.W) out[p] = dx[p];
         __syncthreads();  // LDS and `next` are reused by the next tile
     }
@@ -363,7 +362,11 @@ __global__ __launch_bounds__(kThreads) void reduce_a_kernel(const float *__restr
 }
 
 size_t prep_lds_bytes(int R, int S, int K) { return (size_t)prep_layout(R, S, K).total; }
-size_t dx_lds_bytes(int R, int K) { return sizeof(float) * ((size_t)lds_floats(TW, TH, R) + 3 * K); }
+size_t dx_lds_bytes(int R, int K)
+{
+    (void)K;
+    return sizeof(float) * (size_t)lds_floats(DXW, DXH, R);
+}
 
 }  // namespace
 
@@ -375,15 +378,15 @@ void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, 
         b = 1;
     *prep = b > 0 ? b : 1;
     b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_dx_kernel, kThreads, dx_lds_bytes(radius, num_kernels)) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_dx_kernel, DXT, dx_lds_bytes(radius, num_kernels)) !=
         hipSuccess)
         b = 1;
     *dx = b > 0 ? b : 1;
 }
 
 hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDesc *d_imgs, const uint2 *d_tiles,
-                           int num_tiles, int *d_counter, int radius, int grid_prep, int grid_dx, float *partial,
-                           float *dkernel, hipStream_t stream)
+                           int num_tiles, const uint2 *dx_tiles, int num_dx_tiles, int *d_counter, int radius,
+                           int grid_prep, int grid_dx, float *partial, float *dkernel, hipStream_t stream)
 {
     const int K = ks.D * ks.S, NW = adj.num_weights;
     if (ks.S > kMaxScales) return hipErrorInvalidValue;
@@ -393,8 +396,8 @@ hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDe
     hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(kThreads), 0, stream, partial, rows, 0.0f);
     hipLaunchKernelGGL(bwd_prep_kernel, dim3(grid_prep), dim3(kThreads), prep_lds_bytes(radius, ks.S, K), stream, ks,
                        adj, d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
-    hipLaunchKernelGGL(bwd_dx_kernel, dim3(grid_dx), dim3(kThreads), dx_lds_bytes(radius, K), stream, ks, adj, d_imgs,
-                       d_tiles, num_tiles, d_counter, radius);
+    hipLaunchKernelGGL(bwd_dx_kernel, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, adj, d_imgs,
+                       dx_tiles, num_dx_tiles, d_counter, radius);
     if (dkernel) {
         float *e_k = partial + rows;  // K floats after the rows
         hipLaunchKernelGGL(reduce_e_kernel, dim3((K + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, partial,

@@ -39,7 +39,7 @@ struct TileShape {
     int tw, th, px, body;
 };
 constexpr int kThreads = 256;
-constexpr int kNumShapes = 8;
+constexpr int kNumShapes = 10;
 constexpr TileShape kShapes[kNumShapes] = {
     {32, 32, 4, 1},  // 0: wave covers 32x8 px
     {32, 32, 4, 0},  // 1: same, window loop
@@ -49,6 +49,8 @@ constexpr TileShape kShapes[kNumShapes] = {
     {64, 32, 8, 1},  // 5: wave covers 64x8 px
     {64, 16, 4, 0},  // 6: wave covers 64x4 px, window loop
     {32, 16, 2, 0},  // 7: wave covers 32x4 px, window loop
+    {32, 64, 8, 1},  // 8: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
+    {16, 64, 4, 1},  // 9: wave covers 16x16 px
 };
 constexpr int kMaxRadius = 31;
 
@@ -96,14 +98,16 @@ hipError_t launch_forward_batched(int shape, bool exact, const DevKset &ks, cons
 // Work-groups of one shape that fit on a CU at this radius (LDS bound).
 int blocks_per_cu(int shape, int radius);
 
-// Adjoint (lpa_bwd.hip).  Tiles use shape 0 (32 x 32, 4 px per lane); the
-// tile queue and descriptors are laid out as for the batched forward.
-//   partial: [grid][num_weights + K] floats, one row per work-group of the
-//            first pass, written (not accumulated) by that pass
+// Adjoint (lpa_bwd.hip).  The first pass walks 32 x 32 tiles (d_tiles), the
+// dx pass 64 x 64 tiles (dx_tiles); both queues are laid out as for the
+// batched forward ({image, (tile_y << 16) | tile_x}).
+//   partial: [grid_prep][num_weights + K] floats (+K for the E sums), one row
+//            per work-group of the first pass, written by that pass
 //   dkernel: += reduction of the rows plus the variance term (may be null)
+constexpr int kBwdTile = 32, kDxTile = 64;
 hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDesc *d_imgs, const uint2 *d_tiles,
-                           int num_tiles, int *d_counter, int radius, int grid_prep, int grid_dx, float *partial,
-                           float *dkernel, hipStream_t stream);
+                           int num_tiles, const uint2 *dx_tiles, int num_dx_tiles, int *d_counter, int radius,
+                           int grid_prep, int grid_dx, float *partial, float *dkernel, hipStream_t stream);
 // Work-groups per CU of the two tile passes.
 void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, int *dx);
 

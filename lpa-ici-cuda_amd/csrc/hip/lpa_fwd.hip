@@ -216,6 +216,8 @@ hipError_t launch_forward_plain(int shape, bool exact, const DevKset &ks, const 
     case 5: return plain_dispatch<5>(exact, ks, img, radius, stream);
     case 6: return plain_dispatch<6>(exact, ks, img, radius, stream);
     case 7: return plain_dispatch<7>(exact, ks, img, radius, stream);
+    case 8: return plain_dispatch<8>(exact, ks, img, radius, stream);
+    case 9: return plain_dispatch<9>(exact, ks, img, radius, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -233,6 +235,8 @@ hipError_t launch_forward_batched(int shape, bool exact, const DevKset &ks, cons
     case 5: return batched_dispatch<5>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
     case 6: return batched_dispatch<6>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
     case 7: return batched_dispatch<7>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 8: return batched_dispatch<8>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case 9: return batched_dispatch<9>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -248,6 +252,8 @@ int blocks_per_cu(int shape, int radius)
     case 5: return occupancy<5>(radius);
     case 6: return occupancy<6>(radius);
     case 7: return occupancy<7>(radius);
+    case 8: return occupancy<8>(radius);
+    case 9: return occupancy<9>(radius);
     default: return 1;
     }
 }
