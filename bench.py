@@ -60,21 +60,36 @@ def make_image(w, h, seed, sigma, noise_seed):
     return lpa_ici.make_noisy(lpa_ici.synth_image(w, h, seed), sigma, noise_seed)
 
 
-def plain_leg(ks, x_host, steps, warmup, stream, label):
+def executed_taps(ks, maps, ring):
+    """Taps the hardware adds: the walk's nonzero taps, or for the ring form
+    only the ring taps of scales 0..min(s*+1, S-1)."""
+    if not ring:
+        return useful_taps(maps, ks.runs()["nnz"], ks.S)
+    r = ks.ring_taps_per_kernel()
+    return useful_taps(maps, r, ks.S)
+
+
+def plain_leg(ks, x_host, steps, warmup, stream, label, flags=0):
     x = torch.from_numpy(x_host).cuda()
     y = torch.empty_like(x)
     maps = torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda")
-    plan = lpa_ici.Plan(ks, [x], [y], [maps], SIGMA, GAMMA, flags=lpa_ici.FWD_PLAIN)
+    plan = lpa_ici.Plan(ks, [x], [y], [maps], SIGMA, GAMMA, flags=lpa_ici.FWD_PLAIN | flags)
     with torch.cuda.stream(stream):
         ms = timed(lambda: plan.run(stream), steps, warmup, stream)
     taps = useful_taps(maps, ks.runs()["nnz"], ks.S)
+    ring = ks.info()["ring_form"] == 1 and not (flags & lpa_ici.FWD_WEIGHTED)
+    ex = executed_taps(ks, maps, ring)
     npx = x.numel()
     tflops = 2.0 * taps / (ms * 1e-3) / 1e12
+    # executed: one add (ring sums) or one FMA (weighted) per executed tap
+    ex_tflops = (1.0 if ring else 2.0) * ex / (ms * 1e-3) / 1e12
     log(f"[{label}] {x.shape[1]}x{x.shape[0]}: {ms:.3f} ms/pass, {npx / ms / 1e3:.1f} Mpx/s, "
-        f"{taps / npx:.0f} useful taps/px, {tflops:.2f} TFLOP/s")
+        f"{taps / npx:.0f} useful taps/px ({tflops:.2f} TFLOP/s algorithmic), {ex / npx:.0f} executed taps/px "
+        f"({ex_tflops:.2f} TFLOP/s executed, {'ring sums' if ring else 'weighted FMA'})")
     plan.close()
     return {"ms": ms, "mpx_s": npx / ms / 1e3, "useful_taps": taps, "taps_per_px": taps / npx, "tflops": tflops,
-            "y": y, "x": x}
+            "executed_taps_per_px": ex / npx, "executed_tflops": ex_tflops,
+            "walk": "ring sums" if ring else "weighted FMA", "y": y, "x": x}
 
 
 def batched_leg(ks, steps, warmup, stream, n_images=256, seed=3):
@@ -247,8 +262,18 @@ def main():
                      "unit": "TFLOP/s", "frac": round(main_leg["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4),
                      "traffic": None,
                      "algorithmic_flop_per_launch": 2 * main_leg["useful_taps"],
-                     "useful_taps_per_px": round(main_leg["taps_per_px"], 1)},
+                     "useful_taps_per_px": round(main_leg["taps_per_px"], 1),
+                     "walk": main_leg["walk"],
+                     "executed_taps_per_px": round(main_leg["executed_taps_per_px"], 1),
+                     "executed_tflops": round(main_leg["executed_tflops"], 3),
+                     "executed_frac": round(main_leg["executed_tflops"] / MI355X_FP32_PEAK_TFLOPS, 4)},
     }
+    if main_leg["walk"] == "ring sums":
+        w = plain_leg(ks, x_host, args.steps, args.warmup, stream, "plain symmetric weighted",
+                      flags=lpa_ici.FWD_WEIGHTED)
+        rec["weighted_walk"] = {"mpx_s": round(w["mpx_s"], 2), "ms": round(w["ms"], 4),
+                                "tflops": round(w["tflops"], 3),
+                                "frac": round(w["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4)}
     if not args.skip_cpu:
         rec["cpu_baseline"] = cpu_baseline(x_host)
         log(f"[cpu] {rec['cpu_baseline']}")

@@ -68,7 +68,11 @@ typedef struct lpa_kset_info {
     int num_runs;      /* row runs over all kernels */
     int num_weights;   /* run weights over all kernels (interior zeros included) */
     int num_nonzero;   /* nonzero taps over all kernels */
-    int max_radius;    /* largest rows/2 or cols/2 */
+    int max_radius;    /* largest |row| or |column| offset of a nonzero tap */
+    int ring_form;     /* 1: every kernel has uniform weights and each scale's support
+                          contains the previous one's, so the fast walk builds each
+                          scale's estimate from the previous sum plus the new ring */
+    int num_ring_taps; /* taps over all rings (ring form only) */
 } lpa_kset_info;
 lpa_status lpa_kset_get_info(lpa_kset_t ks, lpa_kset_info* info);
 
@@ -102,7 +106,8 @@ enum {
     LPA_FWD_PLAIN = 1u,  /* one launch per image (the plain form) */
     LPA_FWD_EXACT = 2u,  /* separately rounded mul and add in the reference's tap
                             order: the same bits as the CPU v0_0 path (slower) */
-    LPA_FWD_GRAPH = 4u   /* plans only: capture the launch into a hipGraph and replay it */
+    LPA_FWD_GRAPH = 4u,  /* plans only: capture the launch into a hipGraph and replay it */
+    LPA_FWD_WEIGHTED = 8u /* use the weighted run walk even for a ring-form kernel set */
 };
 
 /* Denoise n images.  Without LPA_FWD_PLAIN all n go through one persistent

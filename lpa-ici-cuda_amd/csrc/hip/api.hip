@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -40,9 +41,20 @@ struct lpa_kset {
     int2 *d_tap_pos = nullptr, *d_kern_w = nullptr;
     int *d_tap_kernel = nullptr;
     float *d_sumsq = nullptr;
+    // Ring form (see lpa::DevRings), when every kernel has uniform weights
+    // and nested supports across scales.
+    bool has_rings = false;
+    std::vector<int4> ring_runs;
+    std::vector<int2> ring_kern;
+    std::vector<float> ring_scale;
+    int ring_taps = 0;
+    int4 *d_ring_runs = nullptr;
+    int2 *d_ring_kern = nullptr;
+    float *d_ring_scale = nullptr;
     bool uploaded = false;
     std::mutex upload_mutex;
     lpa::DevKset dev() const { return lpa::DevKset{d_runs, d_weights, d_kern, D, S}; }
+    lpa::DevRings rings() const { return lpa::DevRings{d_ring_runs, d_ring_kern, d_ring_scale}; }
     lpa::DevAdjoint adjoint() const
     {
         return lpa::DevAdjoint{lpa::DevKset{d_runs_flip, d_weights_flip, d_kern, D, S},
@@ -133,6 +145,66 @@ void pack_runs(lpa_kset *ks, const float *dense, const int *offsets, const int *
     }
 }
 
+// Detect the ring form: for every direction, each scale's kernel has one
+// weight value on its support and contains the previous scale's support.
+// Then kernel (d,s) = w_s * indicator(support) and its correlation is w_s
+// times a running sum over the rings support(d,s) \ support(d,s-1).
+void build_rings(lpa_kset *ks, const float *dense, const int *offsets, const int *rows, const int *cols)
+{
+    std::vector<std::vector<std::pair<int, int>>> support(ks->K);
+    for (int k = 0; k < ks->K; ++k) {
+        const float *w = dense + offsets[k];
+        const int R = rows[k], C = cols[k];
+        float value = 0.0f;
+        bool first = true;
+        for (int r = 0; r < R; ++r)
+            for (int c = 0; c < C; ++c) {
+                const float v = w[r * C + c];
+                if (v == 0.0f) continue;
+                if (first) {
+                    value = v;
+                    first = false;
+                } else if (std::memcmp(&v, &value, sizeof v) != 0) {
+                    return;  // weights not uniform
+                }
+                support[k].push_back({r - R / 2, c - C / 2});
+            }
+        ks->ring_scale.push_back(value);
+    }
+    for (int d = 0; d < ks->D; ++d)
+        for (int s = 1; s < ks->S; ++s) {
+            const auto &inner = support[d * ks->S + s - 1], &outer = support[d * ks->S + s];
+            if (!std::includes(outer.begin(), outer.end(), inner.begin(), inner.end())) {
+                ks->ring_scale.clear();
+                return;  // supports not nested
+            }
+        }
+    for (int d = 0; d < ks->D; ++d)
+        for (int s = 0; s < ks->S; ++s) {
+            const int k = d * ks->S + s;
+            std::vector<std::pair<int, int>> ring;
+            if (s == 0) {
+                ring = support[k];
+            } else {
+                const auto &inner = support[k - 1];
+                std::set_difference(support[k].begin(), support[k].end(), inner.begin(), inner.end(),
+                                    std::back_inserter(ring));
+            }
+            const int first_run = (int)ks->ring_runs.size();
+            for (size_t i = 0; i < ring.size();) {
+                size_t j = i + 1;
+                while (j < ring.size() && ring[j].first == ring[i].first && ring[j].second == ring[j - 1].second + 1 &&
+                       (int)(j - i) < lpa::kMaxRun)
+                    ++j;
+                ks->ring_runs.push_back(int4{ring[i].first, ring[i].second, (int)(j - i), 0});
+                i = j;
+            }
+            ks->ring_kern.push_back(int2{first_run, (int)ks->ring_runs.size() - first_run});
+            ks->ring_taps += (int)ring.size();
+        }
+    ks->has_rings = true;
+}
+
 // Per-image [3][K] table: variance, threshold gamma*sqrt(var) and 1/var, in
 // float on the host exactly as the reference forms them
 // (src/v0_0_lpa_ici_2D_naive.cpp:684-699 and :887, :918-919).
@@ -217,6 +289,19 @@ lpa_status ensure_uploaded(lpa_kset *ks)
     }
     if (e == hipSuccess) e = hipMemcpy(ks->d_kern_w, ks->kern_w.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ks->d_sumsq, ks->sumsq.data(), sizeof(float) * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ks->has_rings) {
+        const size_t nrr = std::max<size_t>(1, ks->ring_runs.size());
+        e = hipMalloc(&ks->d_ring_runs, sizeof(int4) * nrr);
+        if (e == hipSuccess) e = hipMalloc(&ks->d_ring_kern, sizeof(int2) * K);
+        if (e == hipSuccess) e = hipMalloc(&ks->d_ring_scale, sizeof(float) * K);
+        if (e == hipSuccess && !ks->ring_runs.empty())
+            e = hipMemcpy(ks->d_ring_runs, ks->ring_runs.data(), sizeof(int4) * ks->ring_runs.size(),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(ks->d_ring_kern, ks->ring_kern.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(ks->d_ring_scale, ks->ring_scale.data(), sizeof(float) * K, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) return hip_fail(e, "kernel set upload");
     ks->uploaded = true;
     return LPA_OK;
@@ -299,17 +384,25 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
     return LPA_OK;
 }
 
+lpa::WalkMode walk_mode(const lpa_kset *ks, unsigned flags)
+{
+    if (flags & LPA_FWD_EXACT) return lpa::kWalkExact;
+    if (ks->has_rings && !(flags & LPA_FWD_WEIGHTED)) return lpa::kWalkRing;
+    return lpa::kWalkFma;
+}
+
 lpa_status plan_launch(lpa_plan *p, hipStream_t stream)
 {
-    const bool exact = (p->flags & LPA_FWD_EXACT) != 0;
+    const lpa::WalkMode mode = walk_mode(p->ks, p->flags);
     const lpa::DevKset dk = p->ks->dev();
+    const lpa::DevRings dr = p->ks->rings();
     if (p->flags & LPA_FWD_PLAIN) {
         for (int i = 0; i < p->n; ++i) {
-            hipError_t e = lpa::launch_forward_plain(p->shape, exact, dk, p->host_desc[i], p->ks->radius, stream);
+            hipError_t e = lpa::launch_forward_plain(p->shape, mode, dk, dr, p->host_desc[i], p->ks->radius, stream);
             if (e != hipSuccess) return hip_fail(e, "forward launch");
         }
     } else {
-        hipError_t e = lpa::launch_forward_batched(p->shape, exact, dk, p->d_desc, p->d_tiles, p->num_tiles,
+        hipError_t e = lpa::launch_forward_batched(p->shape, mode, dk, dr, p->d_desc, p->d_tiles, p->num_tiles,
                                                    p->d_counter, p->ks->radius, p->grid, stream);
         if (e != hipSuccess) return hip_fail(e, "batched forward launch");
     }
@@ -343,6 +436,7 @@ lpa_status lpa_kset_create(int num_dirs, int num_scales, const float *dense, con
     ks->S = num_scales;
     ks->K = K;
     pack_runs(ks, dense, offsets, rows, cols);
+    build_rings(ks, dense, offsets, rows, cols);
     *out = ks;
     lpa::clear_error();
     return LPA_OK;
@@ -355,7 +449,8 @@ void lpa_kset_destroy(lpa_kset_t ks)
     if (ks->d_weights) (void)hipFree(ks->d_weights);
     if (ks->d_kern) (void)hipFree(ks->d_kern);
     for (void *p : {(void *)ks->d_runs_flip, (void *)ks->d_weights_flip, (void *)ks->d_tap_pos,
-                    (void *)ks->d_tap_kernel, (void *)ks->d_kern_w, (void *)ks->d_sumsq})
+                    (void *)ks->d_tap_kernel, (void *)ks->d_kern_w, (void *)ks->d_sumsq, (void *)ks->d_ring_runs,
+                    (void *)ks->d_ring_kern, (void *)ks->d_ring_scale})
         if (p) (void)hipFree(p);
     delete ks;
 }
@@ -369,6 +464,8 @@ lpa_status lpa_kset_get_info(lpa_kset_t ks, lpa_kset_info *info)
     info->num_weights = (int)ks->weights.size();
     info->num_nonzero = ks->num_nonzero;
     info->max_radius = ks->radius;
+    info->ring_form = ks->has_rings ? 1 : 0;
+    info->num_ring_taps = ks->ring_taps;
     return LPA_OK;
 }
 

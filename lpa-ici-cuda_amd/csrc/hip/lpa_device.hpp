@@ -22,6 +22,21 @@ struct DevKset {
     int D, S;
 };
 
+// Ring form of a kernel set whose kernels have uniform weights and nested
+// supports across scales (every scale's support contains the previous
+// one's): kernel (d,s) = scale[k] * (indicator of its support), and the
+// support is the union of the rings (support(d,s) minus support(d,s-1)) up to
+// s.  runs/kern describe the rings as row runs {row, col, len, 0}; a
+// scale's estimate is scale[k] times the running sum over its rings.
+struct DevRings {
+    const int4 *runs;
+    const int2 *kern;
+    const float *scale;
+};
+
+// Arithmetic of the forward walk.
+enum WalkMode { kWalkFma = 0, kWalkExact = 1, kWalkRing = 2 };
+
 // One image of a batch.  tab points at its [3][K] table.
 struct ImgDesc {
     const float *x;
@@ -39,18 +54,14 @@ struct TileShape {
     int tw, th, px, body;
 };
 constexpr int kThreads = 256;
-constexpr int kNumShapes = 10;
+constexpr int kNumShapes = 6;
 constexpr TileShape kShapes[kNumShapes] = {
     {32, 32, 4, 1},  // 0: wave covers 32x8 px
     {32, 32, 4, 0},  // 1: same, window loop
     {64, 16, 4, 1},  // 2: wave covers 64x4 px
     {32, 16, 2, 1},  // 3: wave covers 32x4 px
-    {16, 16, 1, 1},  // 4: one pixel per lane, wave covers 16x4 px
-    {64, 32, 8, 1},  // 5: wave covers 64x8 px
-    {64, 16, 4, 0},  // 6: wave covers 64x4 px, window loop
-    {32, 16, 2, 0},  // 7: wave covers 32x4 px, window loop
-    {32, 64, 8, 1},  // 8: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
-    {16, 64, 4, 1},  // 9: wave covers 16x16 px
+    {64, 32, 8, 1},  // 4: wave covers 64x8 px
+    {32, 64, 8, 1},  // 5: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
 };
 constexpr int kMaxRadius = 31;
 
@@ -90,11 +101,11 @@ struct DevAdjoint {
 };
 
 // Launch entry points (lpa_fwd.hip).
-hipError_t launch_forward_plain(int shape, bool exact, const DevKset &ks, const ImgDesc &img, int radius,
-                                hipStream_t stream);
-hipError_t launch_forward_batched(int shape, bool exact, const DevKset &ks, const ImgDesc *d_imgs,
-                                  const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
-                                  hipStream_t stream);
+hipError_t launch_forward_plain(int shape, WalkMode mode, const DevKset &ks, const DevRings &rings,
+                                const ImgDesc &img, int radius, hipStream_t stream);
+hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, const DevRings &rings,
+                                  const ImgDesc *d_imgs, const uint2 *d_tiles, int num_tiles, int *d_counter,
+                                  int radius, int grid, hipStream_t stream);
 // Work-groups of one shape that fit on a CU at this radius (LDS bound).
 int blocks_per_cu(int shape, int radius);
 

@@ -25,12 +25,13 @@ namespace {
 // One output tile at (x0, y0) of an image.  `tile` is the work-group's LDS
 // buffer of lds_floats(TW, TH, R) floats.  Contains one barrier, reached by
 // every lane.
-template <int TW, int TH, int P, bool EXACT, int BODY>
-__device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks,
+template <int TW, int TH, int P, int MODE, int BODY>
+__device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks, const DevRings &rings,
                                              const float *__restrict__ tab, const float *__restrict__ x,
                                              float *__restrict__ y, uint8_t *__restrict__ map, int W, int H,
                                              int pitch, int x0, int y0, int R)
 {
+    constexpr bool EXACT = MODE == kWalkExact;
     constexpr int LX = TW / P;
     static_assert(LX * TH == kThreads, "tile shape must use 256 lanes");
     const int lp = lds_pitch(TW, R);
@@ -62,6 +63,9 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
     for (int d = 0; d < ks.D; ++d) {
         float lo[P], hi[P], best[P], binv[P];
         int sel[P];
+        float ring_sum[P];  // ring form: running sum of the tile over scales 0..s
+#pragma unroll
+        for (int p = 0; p < P; ++p) ring_sum[p] = 0.0f;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             // Pixels outside the image start with an empty interval.
@@ -80,9 +84,18 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
             if (!__any(alive)) break;
             const int k = d * S + s;
             float acc[P];
+            if constexpr (MODE == kWalkRing) {
+                // yhat_s = w_s * (sum over the support of scale s), built
+                // from the previous scale's sum plus this scale's ring.
+                if (alive) correlate_runs<P, false, 1, false>(lane, lp, rings.runs, nullptr, rings.kern[k], ring_sum);
+                const float w = rings.scale[k];
 #pragma unroll
-            for (int p = 0; p < P; ++p) acc[p] = 0.0f;
-            if (alive) correlate<P, EXACT, BODY>(lane, lp, ks, k, acc);
+                for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;
+            } else {
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p] = 0.0f;
+                if (alive) correlate<P, EXACT, BODY>(lane, lp, ks, k, acc);
+            }
             const float thr = thr_t[k], inv = inv_t[k];
 #pragma unroll
             for (int p = 0; p < P; ++p) {
@@ -117,9 +130,9 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
 // Plain form: one launch per image.  Work-groups are dealt to the 8 XCDs in
 // turn, so block b runs tile (b % 8) * per_xcd + b / 8: consecutive tiles of
 // one XCD are neighbours and share their halos in that XCD's L2.
-template <int SH, bool EXACT>
-__global__ __launch_bounds__(kThreads) void fwd_plain_kernel(DevKset ks, ImgDesc img, int R, int num_tiles,
-                                                             int per_xcd)
+template <int SH, int MODE>
+__global__ __launch_bounds__(kThreads) void fwd_plain_kernel(DevKset ks, DevRings rings, ImgDesc img, int R,
+                                                             int num_tiles, int per_xcd)
 {
     extern __shared__ float tile[];
     constexpr TileShape sh = kShapes[SH];
@@ -127,15 +140,16 @@ __global__ __launch_bounds__(kThreads) void fwd_plain_kernel(DevKset ks, ImgDesc
     const int t = (b & 7) * per_xcd + (b >> 3);
     if (t >= num_tiles) return;  // whole work-group, before any barrier
     const int tx = t % img.tiles_x, ty = t / img.tiles_x;
-    process_tile<sh.tw, sh.th, sh.px, EXACT, sh.body>(tile, ks, img.tab, img.x, img.y, img.map, img.W, img.H, img.pitch,
-                                             tx * sh.tw, ty * sh.th, R);
+    process_tile<sh.tw, sh.th, sh.px, MODE, sh.body>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W, img.H,
+                                                    img.pitch, tx * sh.tw, ty * sh.th, R);
 }
 
 // Batched form: a persistent grid drains one tile queue that spans every
 // image of the batch.  tiles[t] = {image, (tile_y << 16) | tile_x}.  Every
 // work-group leaves the loop once the counter passes num_tiles.
-template <int SH, bool EXACT>
-__global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, const ImgDesc *__restrict__ imgs,
+template <int SH, int MODE>
+__global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, DevRings rings,
+                                                               const ImgDesc *__restrict__ imgs,
                                                                const uint2 *__restrict__ tiles, int num_tiles,
                                                                int *__restrict__ counter, int R)
 {
@@ -149,46 +163,60 @@ __global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, const
         if (t >= num_tiles) break;
         const uint2 e = tiles[t];
         const ImgDesc img = imgs[e.x];
-        process_tile<sh.tw, sh.th, sh.px, EXACT, sh.body>(tile, ks, img.tab, img.x, img.y, img.map, img.W, img.H,
-                                                 img.pitch, (int)(e.y & 0xffffu) * sh.tw, (int)(e.y >> 16) * sh.th,
-                                                 R);
+        process_tile<sh.tw, sh.th, sh.px, MODE, sh.body>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W,
+                                                        img.H, img.pitch, (int)(e.y & 0xffffu) * sh.tw,
+                                                        (int)(e.y >> 16) * sh.th, R);
         __syncthreads();  // the LDS tile and `next` are reused
     }
 }
 
 __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 
-template <int SH>
-hipError_t plain_dispatch(bool exact, const DevKset &ks, const ImgDesc &img, int R, hipStream_t stream)
+template <int SH, int MODE>
+hipError_t plain_launch(const DevKset &ks, const DevRings &rings, const ImgDesc &img, int R, hipStream_t stream)
 {
     constexpr TileShape sh = kShapes[SH];
     const int tiles_y = (img.H + sh.th - 1) / sh.th;
     const int num_tiles = img.tiles_x * tiles_y;
     const int per_xcd = (num_tiles + 7) / 8;
     const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R);
-    if (exact)
-        hipLaunchKernelGGL((fwd_plain_kernel<SH, true>), dim3(8 * per_xcd), dim3(kThreads), lds, stream, ks, img, R,
-                           num_tiles, per_xcd);
-    else
-        hipLaunchKernelGGL((fwd_plain_kernel<SH, false>), dim3(8 * per_xcd), dim3(kThreads), lds, stream, ks, img,
-                           R, num_tiles, per_xcd);
+    hipLaunchKernelGGL((fwd_plain_kernel<SH, MODE>), dim3(8 * per_xcd), dim3(kThreads), lds, stream, ks, rings, img,
+                       R, num_tiles, per_xcd);
     return hipGetLastError();
 }
 
 template <int SH>
-hipError_t batched_dispatch(bool exact, const DevKset &ks, const ImgDesc *imgs, const uint2 *tiles, int num_tiles,
-                            int *counter, int R, int grid, hipStream_t stream)
+hipError_t plain_dispatch(WalkMode mode, const DevKset &ks, const DevRings &rings, const ImgDesc &img, int R,
+                          hipStream_t stream)
+{
+    switch (mode) {
+    case kWalkExact: return plain_launch<SH, kWalkExact>(ks, rings, img, R, stream);
+    case kWalkRing: return plain_launch<SH, kWalkRing>(ks, rings, img, R, stream);
+    default: return plain_launch<SH, kWalkFma>(ks, rings, img, R, stream);
+    }
+}
+
+template <int SH, int MODE>
+hipError_t batched_launch(const DevKset &ks, const DevRings &rings, const ImgDesc *imgs, const uint2 *tiles,
+                          int num_tiles, int *counter, int R, int grid, hipStream_t stream)
 {
     constexpr TileShape sh = kShapes[SH];
     const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R);
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
-    if (exact)
-        hipLaunchKernelGGL((fwd_batched_kernel<SH, true>), dim3(grid), dim3(kThreads), lds, stream, ks, imgs, tiles,
-                           num_tiles, counter, R);
-    else
-        hipLaunchKernelGGL((fwd_batched_kernel<SH, false>), dim3(grid), dim3(kThreads), lds, stream, ks, imgs,
-                           tiles, num_tiles, counter, R);
+    hipLaunchKernelGGL((fwd_batched_kernel<SH, MODE>), dim3(grid), dim3(kThreads), lds, stream, ks, rings, imgs, tiles,
+                       num_tiles, counter, R);
     return hipGetLastError();
+}
+
+template <int SH>
+hipError_t batched_dispatch(WalkMode mode, const DevKset &ks, const DevRings &rings, const ImgDesc *imgs,
+                            const uint2 *tiles, int num_tiles, int *counter, int R, int grid, hipStream_t stream)
+{
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+    switch (mode) {
+    case kWalkExact: return batched_launch<SH, kWalkExact>(ks, rings, imgs, tiles, num_tiles, counter, R, grid, stream);
+    case kWalkRing: return batched_launch<SH, kWalkRing>(ks, rings, imgs, tiles, num_tiles, counter, R, grid, stream);
+    default: return batched_launch<SH, kWalkFma>(ks, rings, imgs, tiles, num_tiles, counter, R, grid, stream);
+    }
 }
 
 template <int SH>
@@ -196,7 +224,7 @@ int occupancy(int R)
 {
     const size_t lds = sizeof(float) * (size_t)lds_floats(kShapes[SH].tw, kShapes[SH].th, R);
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, false>, kThreads, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, kWalkFma>, kThreads, lds) !=
         hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
@@ -204,39 +232,29 @@ int occupancy(int R)
 
 }  // namespace
 
-hipError_t launch_forward_plain(int shape, bool exact, const DevKset &ks, const ImgDesc &img, int radius,
-                                hipStream_t stream)
+hipError_t launch_forward_plain(int shape, WalkMode mode, const DevKset &ks, const DevRings &rings,
+                                const ImgDesc &img, int radius, hipStream_t stream)
 {
     switch (shape) {
-    case 0: return plain_dispatch<0>(exact, ks, img, radius, stream);
-    case 1: return plain_dispatch<1>(exact, ks, img, radius, stream);
-    case 2: return plain_dispatch<2>(exact, ks, img, radius, stream);
-    case 3: return plain_dispatch<3>(exact, ks, img, radius, stream);
-    case 4: return plain_dispatch<4>(exact, ks, img, radius, stream);
-    case 5: return plain_dispatch<5>(exact, ks, img, radius, stream);
-    case 6: return plain_dispatch<6>(exact, ks, img, radius, stream);
-    case 7: return plain_dispatch<7>(exact, ks, img, radius, stream);
-    case 8: return plain_dispatch<8>(exact, ks, img, radius, stream);
-    case 9: return plain_dispatch<9>(exact, ks, img, radius, stream);
+    case 0: return plain_dispatch<0>(mode, ks, rings, img, radius, stream);
+    case 1: return plain_dispatch<1>(mode, ks, rings, img, radius, stream);
+    case 2: return plain_dispatch<2>(mode, ks, rings, img, radius, stream);
+    case 3: return plain_dispatch<3>(mode, ks, rings, img, radius, stream);
+    case 4: return plain_dispatch<4>(mode, ks, rings, img, radius, stream);
+    case 5: return plain_dispatch<5>(mode, ks, rings, img, radius, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_forward_batched(int shape, bool exact, const DevKset &ks, const ImgDesc *d_imgs,
-                                  const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
-                                  hipStream_t stream)
+hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, const DevRings &rings,
+                                  const ImgDesc *d_imgs, const uint2 *d_tiles, int num_tiles, int *d_counter,
+                                  int radius, int grid, hipStream_t stream)
 {
     switch (shape) {
-    case 0: return batched_dispatch<0>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 1: return batched_dispatch<1>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 2: return batched_dispatch<2>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 3: return batched_dispatch<3>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 4: return batched_dispatch<4>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 5: return batched_dispatch<5>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 6: return batched_dispatch<6>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 7: return batched_dispatch<7>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 8: return batched_dispatch<8>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-    case 9: return batched_dispatch<9>(exact, ks, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+#define LPA_B(SH)                                                                                                  \
+    case SH: return batched_dispatch<SH>(mode, ks, rings, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+        LPA_B(0) LPA_B(1) LPA_B(2) LPA_B(3) LPA_B(4) LPA_B(5)
+#undef LPA_B
     default: return hipErrorInvalidValue;
     }
 }
@@ -250,10 +268,6 @@ int blocks_per_cu(int shape, int radius)
     case 3: return occupancy<3>(radius);
     case 4: return occupancy<4>(radius);
     case 5: return occupancy<5>(radius);
-    case 6: return occupancy<6>(radius);
-    case 7: return occupancy<7>(radius);
-    case 8: return occupancy<8>(radius);
-    case 9: return occupancy<9>(radius);
     default: return 1;
     }
 }

@@ -28,19 +28,27 @@ __device__ __forceinline__ float tap(float acc, float w, float v)
 // One run of exactly L taps for the P pixels starting at `src`: L + P - 1
 // tile values and L weights, then L*P FMAs.  Per pixel the taps are added in
 // order along the run.
-template <int L, int P, bool EXACT>
+template <int L, int P, bool EXACT, bool WEIGHTED = true>
 __device__ __forceinline__ void run_taps(const float *__restrict__ src, const float *__restrict__ w, float (&acc)[P])
 {
     float v[L + P - 1];
 #pragma unroll
     for (int q = 0; q < L + P - 1; ++q) v[q] = src[q];
-    float c[L];
+    if constexpr (WEIGHTED) {
+        float c[L];
 #pragma unroll
-    for (int j = 0; j < L; ++j) c[j] = w[j];
+        for (int j = 0; j < L; ++j) c[j] = w[j];
 #pragma unroll
-    for (int j = 0; j < L; ++j)
+        for (int j = 0; j < L; ++j)
 #pragma unroll
-        for (int p = 0; p < P; ++p) acc[p] = tap<EXACT>(acc[p], c[j], v[j + p]);
+            for (int p = 0; p < P; ++p) acc[p] = tap<EXACT>(acc[p], c[j], v[j + p]);
+    } else {
+        (void)w;
+#pragma unroll
+        for (int j = 0; j < L; ++j)
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[p] = acc[p] + v[j + p];
+    }
 }
 
 // Sliding-window body: a window of P+3 tile values moves along the run four
@@ -81,31 +89,31 @@ __device__ __forceinline__ void run_window(const float *__restrict__ src, const 
 // Runs are at most kMaxRun taps long (the packer splits longer ones).
 constexpr int kMaxRun = 32;
 
-// acc[p] += sum over kernel k's runs of w[j] * tile[row, col + p + j], for the
-// P pixels starting at `lane` (the LDS address of the lane's first pixel).
-// Per pixel the taps are added in run order, then in order along the run:
-// the row-major order of the dense kernel.  Each LDS value read feeds up to P
+// acc[p] += sum over the runs kr = {first, count} of w[j] * tile[row, col + p + j]
+// (or of tile[row, col + p + j] alone when !WEIGHTED), for the P pixels
+// starting at `lane` (the LDS address of the lane's first pixel).  Per pixel
+// the taps are added in run order, then in order along the run: the
+// row-major order of the dense kernel.  Each LDS value read feeds up to P
 // FMAs; the run's address costs one add from the lane's base; weights and
 // run records are wave-uniform scalar loads.  BODY 1: the run length is the
 // same for every lane, so a switch picks one fully unrolled body per run (no
 // window shuffling, no loop tail).  BODY 0: the sliding-window loop.
-template <int P, bool EXACT, int BODY = 1>
-__device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp, const DevKset &ks, int k,
-                                          float (&acc)[P])
+template <int P, bool EXACT, int BODY, bool WEIGHTED>
+__device__ __forceinline__ void correlate_runs(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
+                                               const float *__restrict__ weights, int2 kr, float (&acc)[P])
 {
-    const int2 kr = ks.kern[k];
     for (int r = 0; r < kr.y; ++r) {
-        const int4 run = ks.runs[kr.x + r];
+        const int4 run = runs[kr.x + r];
         const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
         const float *__restrict__ src = lane + off;
-        const float *__restrict__ w = ks.weights + run.w;
-        if constexpr (BODY == 0) {
+        const float *__restrict__ w = weights + run.w;
+        if constexpr (BODY == 0 && WEIGHTED) {
             run_window<P, EXACT>(src, w, run.z, acc);
             continue;
         }
         switch (run.z) {
 #define LPA_RUN_CASE(L)                                                                                            \
-    case L: run_taps<L, P, EXACT>(src, w, acc); break;
+    case L: run_taps<L, P, EXACT, WEIGHTED>(src, w, acc); break;
             LPA_RUN_CASE(1) LPA_RUN_CASE(2) LPA_RUN_CASE(3) LPA_RUN_CASE(4) LPA_RUN_CASE(5) LPA_RUN_CASE(6)
             LPA_RUN_CASE(7) LPA_RUN_CASE(8) LPA_RUN_CASE(9) LPA_RUN_CASE(10) LPA_RUN_CASE(11) LPA_RUN_CASE(12)
             LPA_RUN_CASE(13) LPA_RUN_CASE(14) LPA_RUN_CASE(15) LPA_RUN_CASE(16) LPA_RUN_CASE(17) LPA_RUN_CASE(18)
@@ -116,6 +124,14 @@ __device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp
         default: break;
         }
     }
+}
+
+// Weighted correlation of kernel k of a set (see correlate_runs).
+template <int P, bool EXACT, int BODY = 1>
+__device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp, const DevKset &ks, int k,
+                                          float (&acc)[P])
+{
+    correlate_runs<P, EXACT, BODY, true>(lane, lp, ks.runs, ks.weights, ks.kern[k], acc);
 }
 
 // Stage the (TW + 2R) x (TH + 2R) window whose top-left image pixel is

@@ -10,11 +10,12 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (DEFAULT_KERNELS, FWD_EXACT, FWD_GRAPH, FWD_PLAIN, GAUSSIAN_M20_KERNELS, KERNEL_DIR, LIB_PATH,
+from ._lib import (DEFAULT_KERNELS, FWD_EXACT, FWD_GRAPH, FWD_PLAIN, FWD_WEIGHTED, GAUSSIAN_M20_KERNELS, KERNEL_DIR, LIB_PATH,
                    LpaError, LpaImage, check, lib)
 
 __all__ = [
     "DEFAULT_KERNELS", "GAUSSIAN_M20_KERNELS", "KERNEL_DIR", "LIB_PATH", "FWD_PLAIN", "FWD_EXACT", "FWD_GRAPH",
+    "FWD_WEIGHTED",
     "LpaError", "KernelSet", "Plan", "load_kernels", "kernel_variances", "make_noisy", "psnr", "synth_image",
     "load_image", "write_hdr", "denoise", "backward", "image_struct", "tile_configs", "lib",
 ]
@@ -82,6 +83,19 @@ class KernelSet:
                  "nnz", "sumsq"]
         check(lib().lpa_kset_get_runs(self.handle, *[_ptr(out[n]) for n in order]))
         out["weights"] = out["weights"][:NW]
+        return out
+
+    def ring_taps_per_kernel(self):
+        """Taps of each kernel's ring (ring form only): the new support of
+        scale s over scale s-1, so a walk to scale s adds rings 0..s."""
+        out = np.zeros(self.K, np.int64)
+        for d in range(self.D):
+            prev = set()
+            for s in range(self.S):
+                k = self.kernel(d, s)
+                pos = {(r - k.shape[0] // 2, c - k.shape[1] // 2) for r, c in zip(*np.nonzero(k))}
+                out[d * self.S + s] = len(pos - prev)
+                prev = pos
         return out
 
     def set_tile_config(self, index):

@@ -78,12 +78,15 @@ def test_exact_cameraman_bitexact(ks):
     assert np.array_equal(maps, mo)
 
 
-def test_fast_parity_config1_512(ks):
-    """Config 1: one 512x512 synthetic image, sigma 20/255, gamma 2."""
+@pytest.mark.parametrize("flags,label", [(0, "ring sums"), (lpa_ici.FWD_WEIGHTED, "weighted FMA")])
+def test_fast_parity_config1_512(ks, flags, label):
+    """Config 1: one 512x512 synthetic image, sigma 20/255, gamma 2; the
+    default set has the ring form, so both fast walks are held to the bar."""
+    assert ks.info()["ring_form"] == 1
     clean = lpa_ici.synth_image(512, 512, 1)
     x = lpa_ici.make_noisy(clean, SIGMA, SEED)
-    y, maps = run(ks, x)
-    check_fast_parity(ks, x, clean, y, maps, "config1 512^2 symmetric")
+    y, maps = run(ks, x, flags=flags)
+    check_fast_parity(ks, x, clean, y, maps, f"config1 512^2 symmetric, {label}")
 
 
 def test_fast_parity_gaussian_m20(ks):
@@ -94,12 +97,13 @@ def test_fast_parity_gaussian_m20(ks):
     check_fast_parity(ksg, x, clean, y, maps, "256x192 m20 gaussian")
 
 
-def test_fast_parity_barbara(ks):
+@pytest.mark.parametrize("flags,label", [(0, "ring sums"), (lpa_ici.FWD_WEIGHTED, "weighted FMA")])
+def test_fast_parity_barbara(ks, flags, label):
     """The reference's published case: PSNR 25.0736 on barbara."""
     clean = lpa_ici.load_image(lpa_ici.KERNEL_DIR + "/../../tests/golden/images/barbara.png")
     x = lpa_ici.make_noisy(clean, SIGMA, SEED)
-    y, maps = run(ks, x)
-    check_fast_parity(ks, x, clean, y, maps, "barbara")
+    y, maps = run(ks, x, flags=flags)
+    check_fast_parity(ks, x, clean, y, maps, f"barbara, {label}")
     assert round(lpa_ici.psnr(clean, y), 4) == 25.0736
 
 

@@ -252,6 +252,21 @@ def test_run_length_round_trip(name, runs, weights, interior):
     assert np.array_equal((s2 * r["sumsq"]).astype(np.float32), ks.variances(SIGMA))
 
 
+def test_ring_form_detection():
+    """The default set: uniform weights, nested supports, 3172 ring taps
+    (6008 nonzeros); the m=(2,0) gaussian set is not in ring form."""
+    info = kernels(SYMMETRIC).info()
+    assert info["ring_form"] == 1 and info["num_ring_taps"] == 3172
+    for d in range(16):
+        prev = set()
+        for s in range(7):
+            k = kernels(SYMMETRIC).kernel(d, s)
+            pos = {(r - k.shape[0] // 2, c - k.shape[1] // 2) for r, c in zip(*np.nonzero(k))}
+            assert prev <= pos and len(set(k[k != 0].tolist())) == 1
+            prev = pos
+    assert kernels("lpa_kernels_m_2_0_d_16_h12_1_2_4_8_16_24_32_gaussian.txt").info()["ring_form"] == 0
+
+
 def test_kset_rejects_radius_over_31():
     """The reference exits above a 63x63 kernel (src/v4_1_lpa_ici_2D_sparse_shared.cu:1189-1195);
     here it is LPA_ETOOBIG, and a non-positive size is LPA_EINVAL."""
