@@ -159,6 +159,19 @@ def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
             "fwd_bwd_mpx_s": npx / ms_fb / 1e3}
 
 
+def measured_traffic(walk):
+    """HBM bytes per launch of the 4096^2 forward from the committed rocprofv3
+    counter runs (profiles/forward_plain_4096_symmetric.md), or None."""
+    key = {"ring sums": "ring_sums", "weighted FMA": "weighted_fma"}[walk]
+    try:
+        for line in open(os.path.join(REPO, "profiles", "forward_plain_4096_symmetric.traffic")):
+            if line.startswith(key + ":"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(x_host, crop=640):
     """The reference's v1_0 CPU code (fastest of its two CPU versions) on a
     crop of the workload image, one core; our restatement if the reference
@@ -260,7 +273,7 @@ def main():
                    "tile_configs": lpa_ici.tile_configs()},
         "roofline": {"bound": "valu", "achieved": round(main_leg["tflops"], 3), "peak": MI355X_FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(main_leg["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(main_leg["walk"]) if args.size == 4096 else None,
                      "algorithmic_flop_per_launch": 2 * main_leg["useful_taps"],
                      "useful_taps_per_px": round(main_leg["taps_per_px"], 1),
                      "walk": main_leg["walk"],
