@@ -14,8 +14,10 @@ carries
                 plain (one launch per image) vs one persistent launch vs its
                 hipGraph replay
   fwd_bwd       64 x 512^2 forward + adjoint (dL/dx, dL/dkernel)
-Flags --skip-* drop the side legs; --config5 adds 32 images of 1024..8192 px
-fwd+bwd with peak HBM use.
+  config5       32 images with sides U{1024..8192} (about 680 Mpx), fwd+bwd
+                with the estimates recomputed from the uint8 scale maps, and
+                the peak device memory in use against the 288 GB of HBM
+Flags --skip-* drop the side legs.
 """
 import argparse
 import json
@@ -172,6 +174,60 @@ def measured_traffic(walk):
     return None
 
 
+def config5_leg(ks, steps, warmup, stream, n=32, seed=5):
+    """32 images with sides U{1024..8192}, per-image sigma/gamma as the batched
+    leg: forward (scale maps kept as uint8) + adjoint that recomputes the
+    estimates from the maps; peak HBM use reported."""
+    import ctypes
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(1024, 8193, size=(n, 2))
+    sig = [float(rng.choice([10, 20, 30, 40])) / 255.0 for _ in range(n)]
+    gam = [float(rng.choice([1.5, 2.0, 2.5])) for _ in range(n)]
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.time()
+    xs = [torch.from_numpy(make_image(int(w), int(h), 3000 + i, s, SEED + i)).cuda()
+          for i, ((w, h), s) in enumerate(zip(sides, sig))]
+    npx = sum(x.numel() for x in xs)
+    log(f"[config5] generated {n} images, {npx / 1e6:.1f} Mpx in {time.time() - t0:.1f} s")
+    ys = [torch.empty_like(x) for x in xs]
+    maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda") for x in xs]
+    g = torch.Generator(device="cuda").manual_seed(7)
+    dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
+    dxs = [torch.empty_like(x) for x in xs]
+    dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
+    fwd = lpa_ici.Plan(ks, xs, ys, maps, sig, gam, flags=0)
+    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, s_, g_) for x, y, m, s_, g_ in
+                                   zip(xs, ys, maps, sig, gam)])
+    dy_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dys])
+    dx_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dxs])
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    free_min = [free0]
+
+    def step():
+        fwd.run(stream)
+        lpa_ici.check(lpa_ici.lib().lpa_ici_backward(ks.handle, arr, dy_p, dx_p, ctypes.c_void_p(dk.data_ptr()),
+                                                     n, sp))
+        free_min[0] = min(free_min[0], torch.cuda.mem_get_info()[0])
+
+    with torch.cuda.stream(stream):
+        ms_f = timed(lambda: fwd.run(stream), steps, warmup, stream)
+        ms_fb = timed(step, steps, warmup, stream)
+    torch.cuda.synchronize()
+    free_min[0] = min(free_min[0], torch.cuda.mem_get_info()[0])
+    fwd.close()
+    peak = free0 - free_min[0]
+    log(f"[config5] {n} images {npx / 1e6:.1f} Mpx: fwd {ms_f:.1f} ms ({npx / ms_f / 1e3:.0f} Mpx/s), "
+        f"fwd+bwd {ms_fb:.1f} ms ({npx / ms_fb / 1e3:.0f} Mpx/s); peak device memory in use "
+        f"{peak / 2**30:.1f} GiB of {total / 2**30:.0f} GiB")
+    return {"images": n, "mpx": round(npx / 1e6, 1), "fwd_ms": ms_f, "fwd_mpx_s": npx / ms_f / 1e3,
+            "fwd_bwd_ms": ms_fb, "fwd_bwd_mpx_s": npx / ms_fb / 1e3, "peak_hbm_gib": round(peak / 2**30, 2),
+            "hbm_total_gib": round(total / 2**30, 1),
+            "torch_peak_allocated_gib": round(torch.cuda.max_memory_allocated() / 2**30, 2),
+            "scale_maps_gib": round(npx * ks.D / 2**30, 2)}
+
+
 def cpu_baseline(x_host, crop=640):
     """The reference's v1_0 CPU code (fastest of its two CPU versions) on a
     crop of the workload image, one core; our restatement if the reference
@@ -220,6 +276,7 @@ def main():
     ap.add_argument("--skip-gaussian", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-fwdbwd", action="store_true")
+    ap.add_argument("--skip-config5", action="store_true")
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (the reference kernel for the HBM byte counters)")
@@ -306,6 +363,9 @@ def main():
     if not args.skip_fwdbwd:
         fb = fwd_bwd_leg(ks, max(3, args.steps // 4), 1, stream)
         rec["fwd_bwd"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in fb.items()}
+    if not args.skip_config5:
+        c5 = config5_leg(ks, 2, 1, stream)
+        rec["config5"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in c5.items()}
     rec["host"] = platform.node()
     print(json.dumps(rec), flush=True)
 

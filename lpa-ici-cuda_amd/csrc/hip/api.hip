@@ -549,15 +549,15 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     const int K = ks->K, NW = (int)ks->weights.size();
     // Tiles and tables as for the batched forward, shape 0.
     std::vector<uint2> tiles, dx_tiles;
-    auto add_tiles = [&](std::vector<uint2> &v, int edge) {
+    auto add_tiles = [&](std::vector<uint2> &v, int tw, int th) {
         for (int i = 0; i < n; ++i) {
-            const int tx = (imgs[i].width + edge - 1) / edge, ty = (imgs[i].height + edge - 1) / edge;
+            const int tx = (imgs[i].width + tw - 1) / tw, ty = (imgs[i].height + th - 1) / th;
             for (int y = 0; y < ty; ++y)
                 for (int x = 0; x < tx; ++x) v.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
         }
     };
-    add_tiles(tiles, lpa::kBwdTile);
-    add_tiles(dx_tiles, lpa::kDxTile);
+    add_tiles(tiles, lpa::kBwdTileW, lpa::kBwdTileH);
+    add_tiles(dx_tiles, lpa::kDxTile, lpa::kDxTile);
     int bp = 1, bd = 1;
     lpa::backward_occupancy(ks->radius, ks->S, K, &bp, &bd);
     const int cus = num_cus();
@@ -572,7 +572,7 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     const size_t off_dxt = off_tiles + up(sizeof(uint2) * tiles.size());
     const size_t off_q = off_dxt + up(sizeof(uint2) * dx_tiles.size());
     const size_t off_part = off_q + up(sizeof(float) * qpx);
-    const size_t bytes = off_part + sizeof(float) * ((size_t)grid_prep * (NW + K) + K);
+    const size_t bytes = off_part + sizeof(float) * ((size_t)grid_prep * 4 * (NW + K) + K);
     const size_t host_bytes = off_q;
     std::vector<unsigned char> host(host_bytes, 0);
     void *mem = nullptr;

@@ -12,10 +12,10 @@
 // Nothing of the forward but y and the uint8 scale maps is kept: yhat is
 // recomputed from the maps here.
 //
-// Pass 1 (prep, one tile at a time): q, the recomputed yhat of the selected
-// scale, E and A.  Per direction the tile's pixels are grouped by selected
-// scale (deterministic ballot order); A is formed with one lane per tap,
-// reducing over the group, into the work-group's own row of `partial`.
+// Pass 1 (one tile at a time): q, the recomputed yhat of the selected scale,
+// E and A, all from one walk over each selected kernel's runs per wave; the
+// wave's A partials are summed across its lanes by a reduce-scatter and
+// added into the wave's own row of `partial`.
 // Pass 2 (dx): per direction and scale, the masked map q*inv*[s_d = s] over
 // tile + halo is correlated with the 180-degree rotated kernel.
 // Pass 3: rows of `partial` are summed and the variance term is added.
@@ -27,41 +27,16 @@
 namespace lpa {
 namespace {
 
-constexpr int TW = 32, TH = 32, P = 4, LX = TW / P;
-constexpr int kTilePx = TW * TH;
 constexpr int kMaxScales = 16;
 
-struct PrepLds {
-    // Offsets (bytes) of each region inside the dynamic LDS block.
-    int x, tab, q, e, sel, list_q, list_b, cnt, eblk, total;
-};
+// Pass-1 tile: 64 x 32 output pixels, 8 adjacent pixels per lane, 256 lanes.
+constexpr int GW = 64, GH = 32, GP = 8, GLX = GW / GP;
+static_assert(GLX * GH == kThreads, "pass-1 tile must use 256 lanes");
+// Run pieces the pass-1 bodies are unrolled for (longer runs are walked in
+// pieces of this length).
+constexpr int kGradPiece = 8;
 
-__host__ __device__ inline PrepLds prep_layout(int R, int S, int K)
-{
-    PrepLds l;
-    int o = 0;
-    l.x = o;
-    o += 4 * lds_floats(TW, TH, R);
-    l.tab = o;
-    o += 4 * 3 * K;
-    l.q = o;
-    o += 4 * kTilePx;
-    l.e = o;
-    o += 4 * kTilePx;
-    l.list_q = o;
-    o += 4 * kTilePx;
-    l.list_b = o;
-    o += 4 * kTilePx;
-    l.sel = o;
-    o += kTilePx;
-    l.cnt = o;
-    o += 4 * 2 * kMaxScales;
-    l.eblk = o;
-    o += 4 * K;
-    l.total = (o + 15) & ~15;
-    (void)S;
-    return l;
-}
+__host__ __device__ inline int grad_tab_offset(int R) { return lds_floats(GW, GH, R); }
 
 __device__ __forceinline__ float wave_sum(float v)
 {
@@ -81,32 +56,89 @@ __global__ void reset_counter2(int *c)
     c[1] = 0;
 }
 
-// Pass 1.  counter[0] feeds the tile queue.
-__global__ __launch_bounds__(kThreads) void bwd_prep_kernel(DevKset ks, DevAdjoint adj,
+// Sum L values over the 64 lanes of a wave and add the L totals into
+// row[0..L).  Reduce-scatter: each halving stage exchanges half of the
+// remaining values with the partner lane across one lane bit, so after
+// log2(Lp) stages (Lp = L rounded up to a power of two) every lane holds one
+// partial, which the remaining lane bits sum; the lowest lane of each group
+// owns total j = lane >> (6 - log2 Lp).  Costs Lp - 1 + 6 - log2 Lp shuffles
+// instead of 6 L.
+template <int L>
+__device__ __forceinline__ void reduce_scatter_add(const float (&v)[L], float *__restrict__ row, int lane)
+{
+    constexpr int B = L <= 1 ? 0 : L <= 2 ? 1 : L <= 4 ? 2 : L <= 8 ? 3 : L <= 16 ? 4 : 5;
+    constexpr int Lp = 1 << B;
+    float w[Lp];
+#pragma unroll
+    for (int i = 0; i < Lp; ++i) w[i] = i < L ? v[i] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+        const int b = 32 >> k;
+        const bool upper = (lane & b) != 0;
+        const int half = Lp >> (k + 1);
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const float give = upper ? w[i] : w[i + half];
+            const float keep = upper ? w[i + half] : w[i];
+            w[i] = keep + __shfl_xor(give, b, 64);
+        }
+    }
+#pragma unroll
+    for (int b = 32 >> B; b > 0; b >>= 1) w[0] += __shfl_xor(w[0], b, 64);
+    const int j = lane >> (6 - B);
+    if ((lane & ((1 << (6 - B)) - 1)) == 0 && j < L) row[j] += w[0];
+}
+
+// One run piece of L taps for the lane's GP pixels: yacc[p] += sum_j w[j] x,
+// and the wave's A partials for the piece's taps, sum_p m[p] x[p + j], added
+// into the wave's row at the piece's first run weight.
+template <int L>
+__device__ __forceinline__ void grad_piece(const float *__restrict__ src, const float *__restrict__ w,
+                                           const float (&m)[GP], float (&yacc)[GP], float *__restrict__ row, int lane)
+{
+    float v[L + GP - 1];
+#pragma unroll
+    for (int q = 0; q < L + GP - 1; ++q) v[q] = src[q];
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        const float c = w[j];
+#pragma unroll
+        for (int p = 0; p < GP; ++p) yacc[p] = __builtin_fmaf(c, v[j + p], yacc[p]);
+    }
+    float part[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        float a = m[0] * v[j];
+#pragma unroll
+        for (int p = 1; p < GP; ++p) a = __builtin_fmaf(m[p], v[j + p], a);
+        part[j] = a;
+    }
+    reduce_scatter_add<L>(part, row, lane);
+}
+
+// Pass 1.  For every tile: q = g / sum_d inv_d (written to the q map for pass
+// 2); then, per direction and for every scale selected by some pixel of the
+// wave, one walk over that kernel's runs recomputes yhat for the pixels that
+// selected it and forms the wave's A partials with the masked coefficients
+// m_p = [s_d(p) = s] q_p inv, and the wave's E partial.  Each wave owns one
+// row of `partial` ([rows][NW + K]) and read-modify-writes it; no atomics, no
+// barrier after the tile is staged.
+__global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoint adj,
                                                             const BwdDesc *__restrict__ imgs,
                                                             const uint2 *__restrict__ tiles, int num_tiles,
                                                             int *__restrict__ counter, int R,
                                                             float *__restrict__ partial)
 {
-    extern __shared__ __align__(16) unsigned char lds[];
+    extern __shared__ __align__(16) float lds_f[];
     __shared__ int next;
     const int S = ks.S, D = ks.D, K = D * S;
-    const PrepLds L = prep_layout(R, S, K);
-    float *xt = (float *)(lds + L.x);
-    float *tab = (float *)(lds + L.tab);
-    float *qt = (float *)(lds + L.q);
-    float *et = (float *)(lds + L.e);
-    float *lq = (float *)(lds + L.list_q);
-    int *lb = (int *)(lds + L.list_b);
-    uint8_t *sel = lds + L.sel;
-    int *cnt = (int *)(lds + L.cnt);  // [0, S): counts, [kMaxScales, +S): offsets
-    float *eblk = (float *)(lds + L.eblk);
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int lp = lds_pitch(TW, R);
+    const int lp = lds_pitch(GW, R);
+    float *xt = lds_f;
+    float *tab = lds_f + grad_tab_offset(R);
+    const int tid = threadIdx.x, lane = tid & 63;
     const int NW = adj.num_weights;
-    float *__restrict__ row = partial + (size_t)blockIdx.x * (NW + K);
+    float *__restrict__ row = partial + ((size_t)blockIdx.x * (kThreads / 64) + (tid >> 6)) * (NW + K);
 
-    for (int k = tid; k < K; k += kThreads) eblk[k] = 0.0f;
     for (;;) {
         if (tid == 0) next = atomicAdd(counter, 1);
         __syncthreads();
@@ -115,134 +147,76 @@ __global__ __launch_bounds__(kThreads) void bwd_prep_kernel(DevKset ks, DevAdjoi
         const uint2 te = tiles[t];
         const BwdDesc bd = imgs[te.x];
         const ImgDesc &img = bd.img;
-        const int x0 = (int)(te.y & 0xffffu) * TW, y0 = (int)(te.y >> 16) * TH;
+        const int x0 = (int)(te.y & 0xffffu) * GW, y0 = (int)(te.y >> 16) * GH;
         for (int i = tid; i < 3 * K; i += kThreads) tab[i] = img.tab[i];
-        stage_tile<TW, TH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
+        stage_tile<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
         __syncthreads();
 
         const float *inv_t = tab + 2 * K;
-        const int tx = tid % LX, ty = tid / LX;
-        const int px0 = x0 + tx * P, py = y0 + ty;
-        const float *lane_x = xt + (ty + R) * lp + tx * P + R;
+        const int tx = tid % GLX, ty = tid / GLX;
+        const int px0 = x0 + tx * GP, py = y0 + ty;
+        const float *lane_x = xt + (ty + R) * lp + tx * GP + R;
         const size_t plane = (size_t)img.W * img.H;
         const size_t pix = (size_t)py * img.W + px0;
-        bool inside[P];
-        float q[P], yv[P];
+        bool inside[GP];
+        float q[GP], yv[GP];
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
+        for (int p = 0; p < GP; ++p) {
             inside[p] = py < img.H && px0 + p < img.W;
             float wt = 0.0f;
-            for (int d = 0; d < D; ++d) {
-                const int s = inside[p] ? min((int)img.map[(size_t)d * plane + pix + p], S - 1) : 0;
-                wt += inv_t[d * S + s];
-            }
+            if (inside[p])
+                for (int d = 0; d < D; ++d) wt += inv_t[d * S + min((int)img.map[(size_t)d * plane + pix + p], S - 1)];
             const size_t gi = (size_t)py * img.pitch + px0 + p;
             q[p] = inside[p] ? bd.g[gi] / wt : 0.0f;
             yv[p] = inside[p] ? img.y[gi] : 0.0f;
-            qt[ty * TW + tx * P + p] = q[p];
-            if (inside[p]) bd.q[(size_t)py * img.W + px0 + p] = q[p];
+            if (inside[p]) bd.q[pix + p] = q[p];
         }
 
         for (int d = 0; d < D; ++d) {
-            int sd[P];
+            int sd[GP];
 #pragma unroll
-            for (int p = 0; p < P; ++p) sd[p] = inside[p] ? img.map[(size_t)d * plane + pix + p] : 255;
-            // Recompute yhat_d at the selected scale; a wave runs a scale
-            // only if one of its pixels selected it.
-            float yh[P];
-#pragma unroll
-            for (int p = 0; p < P; ++p) yh[p] = 0.0f;
+            for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? img.map[(size_t)d * plane + pix + p] : 255;
             for (int s = 0; s < S; ++s) {
                 bool want = false;
 #pragma unroll
-                for (int p = 0; p < P; ++p) want |= sd[p] == s;
+                for (int p = 0; p < GP; ++p) want |= sd[p] == s;
                 if (!__any(want)) continue;
-                float acc[P];
-#pragma unroll
-                for (int p = 0; p < P; ++p) acc[p] = 0.0f;
-                if (want) correlate<P, false>(lane_x, lp, ks, d * S + s, acc);
-#pragma unroll
-                for (int p = 0; p < P; ++p)
-                    if (sd[p] == s) yh[p] = acc[p];
-            }
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const int i = ty * TW + tx * P + p;
-                const int si = sd[p] < S ? sd[p] : 0;
-                et[i] = inside[p] ? q[p] * inv_t[d * S + si] * (yh[p] - yv[p]) : 0.0f;
-                sel[i] = (uint8_t)sd[p];
-            }
-            __syncthreads();
-
-            // Group the tile's pixels by selected scale: counts, offsets,
-            // then placement in ballot order.  Wave w owns scales w, w+4, ...
-            for (int s = wave; s < S; s += kThreads / 64) {
-                int c = 0;
-                float esum = 0.0f;
-                for (int ch = 0; ch < kTilePx / 64; ++ch) {
-                    const int i = ch * 64 + lane;
-                    const bool m = sel[i] == s;
-                    c += __popcll(__ballot(m));
-                    esum += m ? et[i] : 0.0f;
-                }
-                esum = wave_sum(esum);
-                if (lane == 0) {
-                    cnt[s] = c;
-                    eblk[d * S + s] += esum;
-                }
-            }
-            __syncthreads();
-            if (tid == 0) {
-                int o = 0;
-                for (int s = 0; s < S; ++s) {
-                    cnt[kMaxScales + s] = o;
-                    o += cnt[s];
-                }
-            }
-            __syncthreads();
-            for (int s = wave; s < S; s += kThreads / 64) {
-                int pos = cnt[kMaxScales + s];
-                const float inv = inv_t[d * S + s];
-                for (int ch = 0; ch < kTilePx / 64; ++ch) {
-                    const int i = ch * 64 + lane;
-                    const bool m = sel[i] == s;
-                    const unsigned long long mask = __ballot(m);
-                    if (m) {
-                        const int at = pos + __popcll(mask & ((1ull << lane) - 1ull));
-                        lq[at] = qt[i] * inv;
-                        lb[at] = ((i / TW) + R) * lp + (i % TW) + R;
-                    }
-                    pos += __popcll(mask);
-                }
-            }
-            __syncthreads();
-
-            // A_k[t] += sum over the group of c_d(p) x[p+t], one lane per tap.
-            for (int s = 0; s < S; ++s) {
-                const int n = cnt[s];
-                if (n == 0) continue;
                 const int k = d * S + s;
-                const int2 kw = adj.kern_w[k];
-                const float *gq = lq + cnt[kMaxScales + s];
-                const int *gb = lb + cnt[kMaxScales + s];
-                for (int j = tid; j < kw.y; j += kThreads) {
-                    const int2 tp = adj.tap_pos[kw.x + j];
-                    const int off = tp.x * lp + tp.y;
-                    float a0 = 0.0f, a1 = 0.0f;
-                    int e = 0;
-                    for (; e + 2 <= n; e += 2) {
-                        a0 = __builtin_fmaf(gq[e], xt[gb[e] + off], a0);
-                        a1 = __builtin_fmaf(gq[e + 1], xt[gb[e + 1] + off], a1);
-                    }
-                    if (e < n) a0 = __builtin_fmaf(gq[e], xt[gb[e] + off], a0);
-                    row[kw.x + j] += a0 + a1;
+                const float inv = inv_t[k];
+                float m[GP], yacc[GP];
+#pragma unroll
+                for (int p = 0; p < GP; ++p) {
+                    m[p] = sd[p] == s ? q[p] * inv : 0.0f;
+                    yacc[p] = 0.0f;
                 }
+                const int2 kr = ks.kern[k];
+                for (int r = 0; r < kr.y; ++r) {
+                    const int4 run = ks.runs[kr.x + r];
+                    for (int j0 = 0; j0 < run.z; j0 += kGradPiece) {
+                        const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y + j0);
+                        const float *src = lane_x + off;
+                        const float *__restrict__ w = ks.weights + run.w + j0;
+                        float *__restrict__ rrow = row + run.w + j0;
+                        switch (min(kGradPiece, run.z - j0)) {
+#define LPA_GRAD_CASE(L)                                                                                           \
+    case L: grad_piece<L>(src, w, m, yacc, rrow, lane); break;
+                            LPA_GRAD_CASE(1) LPA_GRAD_CASE(2) LPA_GRAD_CASE(3) LPA_GRAD_CASE(4) LPA_GRAD_CASE(5)
+                            LPA_GRAD_CASE(6) LPA_GRAD_CASE(7) LPA_GRAD_CASE(8)
+#undef LPA_GRAD_CASE
+                        default: break;
+                        }
+                    }
+                }
+                // E_k: sum over the pixels that selected s of c (yhat - y).
+                float e = 0.0f;
+#pragma unroll
+                for (int p = 0; p < GP; ++p) e = __builtin_fmaf(m[p], yacc[p] - yv[p], e);
+                e = wave_sum(e);
+                if (lane == 0) row[NW + k] += e;
             }
         }
-        __syncthreads();  // LDS regions and `next` are reused by the next tile
+        __syncthreads();  // the LDS tile, the table and `next` are reused by the next tile
     }
-    __syncthreads();
-    for (int k = tid; k < K; k += kThreads) row[NW + k] = eblk[k];
 }
 
 // Pass 2 (dx), on 64 x 64 tiles with 1024 lanes (4 px per lane).  Each lane
@@ -361,7 +335,7 @@ __global__ __launch_bounds__(kThreads) void reduce_a_kernel(const float *__restr
     dkernel[i] += s - 2.0f * weights[i] * e_k[k] / sumsq[k];
 }
 
-size_t prep_lds_bytes(int R, int S, int K) { return (size_t)prep_layout(R, S, K).total; }
+size_t grad_lds_bytes(int R, int K) { return sizeof(float) * ((size_t)grad_tab_offset(R) + 3 * K); }
 size_t dx_lds_bytes(int R, int K)
 {
     (void)K;
@@ -373,8 +347,9 @@ size_t dx_lds_bytes(int R, int K)
 void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, int *dx)
 {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_prep_kernel, kThreads,
-                                                     prep_lds_bytes(radius, num_scales, num_kernels)) != hipSuccess)
+    (void)num_scales;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_grad_kernel, kThreads,
+                                                     grad_lds_bytes(radius, num_kernels)) != hipSuccess)
         b = 1;
     *prep = b > 0 ? b : 1;
     b = 0;
@@ -390,20 +365,21 @@ hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDe
 {
     const int K = ks.D * ks.S, NW = adj.num_weights;
     if (ks.S > kMaxScales) return hipErrorInvalidValue;
-    if (prep_lds_bytes(radius, ks.S, K) > 65536 || dx_lds_bytes(radius, K) > 65536) return hipErrorInvalidValue;
-    const size_t rows = (size_t)grid_prep * (NW + K);
+    if (grad_lds_bytes(radius, K) > 65536 || dx_lds_bytes(radius, K) > 65536) return hipErrorInvalidValue;
+    const int rows = grid_prep * (kThreads / 64);
+    const size_t row_floats = (size_t)rows * (NW + K);
     hipLaunchKernelGGL(reset_counter2, dim3(1), dim3(1), 0, stream, d_counter);
-    hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(kThreads), 0, stream, partial, rows, 0.0f);
-    hipLaunchKernelGGL(bwd_prep_kernel, dim3(grid_prep), dim3(kThreads), prep_lds_bytes(radius, ks.S, K), stream, ks,
-                       adj, d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
+    hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(kThreads), 0, stream, partial, row_floats, 0.0f);
+    hipLaunchKernelGGL(bwd_grad_kernel, dim3(grid_prep), dim3(kThreads), grad_lds_bytes(radius, K), stream, ks, adj,
+                       d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
     hipLaunchKernelGGL(bwd_dx_kernel, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, adj, d_imgs,
                        dx_tiles, num_dx_tiles, d_counter, radius);
     if (dkernel) {
-        float *e_k = partial + rows;  // K floats after the rows
+        float *e_k = partial + row_floats;  // K floats after the rows
         hipLaunchKernelGGL(reduce_e_kernel, dim3((K + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, partial,
-                           grid_prep, NW, K, e_k);
+                           rows, NW, K, e_k);
         hipLaunchKernelGGL(reduce_a_kernel, dim3((NW + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, partial,
-                           grid_prep, NW, K, e_k, ks.weights, adj.tap_kernel, adj.sumsq, dkernel);
+                           rows, NW, K, e_k, ks.weights, adj.tap_kernel, adj.sumsq, dkernel);
     }
     return hipGetLastError();
 }

@@ -109,13 +109,13 @@ hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, c
 // Work-groups of one shape that fit on a CU at this radius (LDS bound).
 int blocks_per_cu(int shape, int radius);
 
-// Adjoint (lpa_bwd.hip).  The first pass walks 32 x 32 tiles (d_tiles), the
+// Adjoint (lpa_bwd.hip).  The first pass walks 64 x 32 tiles (d_tiles), the
 // dx pass 64 x 64 tiles (dx_tiles); both queues are laid out as for the
 // batched forward ({image, (tile_y << 16) | tile_x}).
-//   partial: [grid_prep][num_weights + K] floats (+K for the E sums), one row
-//            per work-group of the first pass, written by that pass
+//   partial: [grid_prep * 4][num_weights + K] floats (+K for the E sums), one
+//            row per wave of the first pass, accumulated by that wave
 //   dkernel: += reduction of the rows plus the variance term (may be null)
-constexpr int kBwdTile = 32, kDxTile = 64;
+constexpr int kBwdTileW = 64, kBwdTileH = 32, kDxTile = 64;
 hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDesc *d_imgs, const uint2 *d_tiles,
                            int num_tiles, const uint2 *dx_tiles, int num_dx_tiles, int *d_counter, int radius,
                            int grid_prep, int grid_dx, float *partial, float *dkernel, hipStream_t stream);

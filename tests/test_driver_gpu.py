@@ -1,0 +1,31 @@
+"""The command-line driver end to end on the reference's published case:
+barbara, defaults (sigma 20, gamma 2, seed 20250910, symmetric set) gives
+noisy PSNR 22.1056 and denoised PSNR 25.0736 (README:7, :208), with the
+reference's stdout lines and the three .hdr outputs."""
+import json
+import os
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, PKG, SYMMETRIC, kernel_path
+
+pytestmark = pytest.mark.gpu
+
+
+def test_driver_barbara(tmp_path):
+    out = tmp_path / "barbara_denoised.hdr"
+    r = subprocess.run([os.path.join(PKG, "bin", "lpa_ici_denoise"), "-i",
+                        os.path.join(GOLDEN, "images", "barbara.png"), "-k", kernel_path(SYMMETRIC), "-o", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert "Loaded image with a width of 512 px, a height of 512 px and (imported) 1 channels" in lines
+    assert any(l.startswith("Time taken for directional loops + aggregation:") for l in lines)
+    assert any(l.startswith("Time taken for lpa function execution:") for l in lines)
+    info = json.loads(lines[-1])
+    assert round(info["psnr_noisy"], 4) == 22.1056
+    assert round(info["psnr_denoised"], 4) == 25.0736
+    assert "Calculated PSNR of img_denoised: 25.0736" in r.stdout
+    for f in (out, tmp_path / "gray_image.hdr", tmp_path / "gray_image_noisy.hdr"):
+        assert f.exists() and f.stat().st_size > 512 * 512 * 4
