@@ -38,12 +38,6 @@ constexpr int kGradPiece = 8;
 
 __host__ __device__ inline int grad_tab_offset(int R) { return lds_floats(GW, GH, R); }
 
-__device__ __forceinline__ float wave_sum(float v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 __global__ __launch_bounds__(kThreads) void fill_kernel(float *p, size_t n, float v)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -56,37 +50,88 @@ __global__ void reset_counter2(int *c)
     c[1] = 0;
 }
 
-// Sum L values over the 64 lanes of a wave and add the L totals into
-// row[0..L).  Reduce-scatter: each halving stage exchanges half of the
-// remaining values with the partner lane across one lane bit, so after
-// log2(Lp) stages (Lp = L rounded up to a power of two) every lane holds one
-// partial, which the remaining lane bits sum; the lowest lane of each group
-// owns total j = lane >> (6 - log2 Lp).  Costs Lp - 1 + 6 - log2 Lp shuffles
-// instead of 6 L.
+// Cross-lane helpers.  swap32 / swap16 exchange two registers across lane
+// bit 5 / lane bit 4 (v_permlane32_swap / v_permlane16_swap): afterwards
+// a + b holds, in every lane, its own half plus its partner's copy of the
+// same half.  row_total sums the 16 lanes of each row with four DPP adds.
+__device__ __forceinline__ void swap32(float &a, float &b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ void swap16(float &a, float &b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float row_total(float v)
+{
+    v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp<0x124>(v);  // row_ror:4
+    v += dpp<0x128>(v);  // row_ror:8
+    return v;
+}
+
+// Sum of v over the 64 lanes, in every lane.
+__device__ __forceinline__ float wave_total(float v)
+{
+    float a = v, b = v;
+    swap32(a, b);
+    float c = a + b, d = c;
+    swap16(c, d);
+    return row_total(c + d);
+}
+
+// Sum L (<= 8) values over the 64 lanes of a wave and add the L totals into
+// row[0..L) with fire-and-forget float atomics.  Reduce-scatter: the swaps
+// halve the values per lane across lane bits 5 and 4 without any select, so
+// each lane keeps Lp/4 partial sums (Lp = L rounded up to a power of two),
+// which the row reductions finish; lanes 0, 16, 32 and 48 own
+// j = b5 * Lp/2 + b4 * Lp/4 + i.
 template <int L>
 __device__ __forceinline__ void reduce_scatter_add(const float (&v)[L], float *__restrict__ row, int lane)
 {
-    constexpr int B = L <= 1 ? 0 : L <= 2 ? 1 : L <= 4 ? 2 : L <= 8 ? 3 : L <= 16 ? 4 : 5;
-    constexpr int Lp = 1 << B;
-    float w[Lp];
+    static_assert(L >= 1 && L <= 8, "pieces of at most 8 taps");
+    constexpr int Lp = L <= 1 ? 1 : L <= 2 ? 2 : L <= 4 ? 4 : 8;
+    if constexpr (Lp == 1) {
+        const float t = wave_total(v[0]);
+        if (lane == 0) unsafeAtomicAdd(row, t);
+    } else if constexpr (Lp == 2) {
+        float a = v[0], b = L > 1 ? v[1] : 0.0f;
+        swap32(a, b);
+        float c = a + b, d = c;
+        swap16(c, d);
+        const float t = row_total(c + d);
+        const int j = lane >> 5;
+        if ((lane & 31) == 0 && j < L) unsafeAtomicAdd(row + j, t);
+    } else {
+        float h[Lp / 2];
 #pragma unroll
-    for (int i = 0; i < Lp; ++i) w[i] = i < L ? v[i] : 0.0f;
+        for (int i = 0; i < Lp / 2; ++i) {
+            float a = v[i], b = i + Lp / 2 < L ? v[i + Lp / 2] : 0.0f;
+            swap32(a, b);
+            h[i] = a + b;
+        }
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-        const int b = 32 >> k;
-        const bool upper = (lane & b) != 0;
-        const int half = Lp >> (k + 1);
-#pragma unroll
-        for (int i = 0; i < half; ++i) {
-            const float give = upper ? w[i] : w[i + half];
-            const float keep = upper ? w[i + half] : w[i];
-            w[i] = keep + __shfl_xor(give, b, 64);
+        for (int i = 0; i < Lp / 4; ++i) {
+            float a = h[i], b = h[i + Lp / 4];
+            swap16(a, b);
+            const float t = row_total(a + b);
+            const int j = (lane >> 5) * (Lp / 2) + ((lane >> 4) & 1) * (Lp / 4) + i;
+            if ((lane & 15) == 0 && j < L) unsafeAtomicAdd(row + j, t);
         }
     }
-#pragma unroll
-    for (int b = 32 >> B; b > 0; b >>= 1) w[0] += __shfl_xor(w[0], b, 64);
-    const int j = lane >> (6 - B);
-    if ((lane & ((1 << (6 - B)) - 1)) == 0 && j < L) row[j] += w[0];
 }
 
 // One run piece of L taps for the lane's GP pixels: yacc[p] += sum_j w[j] x,
@@ -120,9 +165,9 @@ __device__ __forceinline__ void grad_piece(const float *__restrict__ src, const 
 // 2); then, per direction and for every scale selected by some pixel of the
 // wave, one walk over that kernel's runs recomputes yhat for the pixels that
 // selected it and forms the wave's A partials with the masked coefficients
-// m_p = [s_d(p) = s] q_p inv, and the wave's E partial.  Each wave owns one
-// row of `partial` ([rows][NW + K]) and read-modify-writes it; no atomics, no
-// barrier after the tile is staged.
+// m_p = [s_d(p) = s] q_p inv, and the wave's E partial.  Each work-group
+// owns one row of `partial` ([grid][NW + K]); its waves add into it with
+// float atomics that return nothing.  No barrier after the tile is staged.
 __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoint adj,
                                                             const BwdDesc *__restrict__ imgs,
                                                             const uint2 *__restrict__ tiles, int num_tiles,
@@ -137,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoi
     float *tab = lds_f + grad_tab_offset(R);
     const int tid = threadIdx.x, lane = tid & 63;
     const int NW = adj.num_weights;
-    float *__restrict__ row = partial + ((size_t)blockIdx.x * (kThreads / 64) + (tid >> 6)) * (NW + K);
+    float *__restrict__ row = partial + (size_t)blockIdx.x * (NW + K);
 
     for (;;) {
         if (tid == 0) next = atomicAdd(counter, 1);
@@ -158,6 +203,10 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoi
         const float *lane_x = xt + (ty + R) * lp + tx * GP + R;
         const size_t plane = (size_t)img.W * img.H;
         const size_t pix = (size_t)py * img.W + px0;
+        const auto gmap = as_global(img.map) + pix;
+        const auto gg = as_global(bd.g) + (size_t)py * img.pitch + px0;
+        const auto gy = as_global(img.y) + (size_t)py * img.pitch + px0;
+        const auto gq = as_global(bd.q) + pix;
         bool inside[GP];
         float q[GP], yv[GP];
 #pragma unroll
@@ -165,17 +214,17 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoi
             inside[p] = py < img.H && px0 + p < img.W;
             float wt = 0.0f;
             if (inside[p])
-                for (int d = 0; d < D; ++d) wt += inv_t[d * S + min((int)img.map[(size_t)d * plane + pix + p], S - 1)];
-            const size_t gi = (size_t)py * img.pitch + px0 + p;
-            q[p] = inside[p] ? bd.g[gi] / wt : 0.0f;
-            yv[p] = inside[p] ? img.y[gi] : 0.0f;
-            if (inside[p]) bd.q[pix + p] = q[p];
+                for (int d = 0; d < D; ++d) wt += inv_t[d * S + min((int)gmap[(size_t)d * plane + p], S - 1)];
+            q[p] = inside[p] ? gg[p] / wt : 0.0f;
+            yv[p] = inside[p] ? gy[p] : 0.0f;
+            if (inside[p]) gq[p] = q[p];
         }
 
         for (int d = 0; d < D; ++d) {
             int sd[GP];
+            const auto gmd = gmap + (size_t)d * plane;
 #pragma unroll
-            for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? img.map[(size_t)d * plane + pix + p] : 255;
+            for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? gmd[p] : 255;
             for (int s = 0; s < S; ++s) {
                 bool want = false;
 #pragma unroll
@@ -211,8 +260,8 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoi
                 float e = 0.0f;
 #pragma unroll
                 for (int p = 0; p < GP; ++p) e = __builtin_fmaf(m[p], yacc[p] - yv[p], e);
-                e = wave_sum(e);
-                if (lane == 0) row[NW + k] += e;
+                e = wave_total(e);
+                if (lane == 0) unsafeAtomicAdd(row + NW + k, e);
             }
         }
         __syncthreads();  // the LDS tile, the table and `next` are reused by the next tile
@@ -260,7 +309,7 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
                 const int r = r0 + 16 * i, c = c0 + 64 * j;
                 const int gy = y0 - R + r, gx = x0 - R + c;
                 const bool in = r < fh && c < fw && gy >= 0 && gy < img.H && gx >= 0 && gx < img.W;
-                qv[i][j] = in ? bd.q[(size_t)gy * img.W + gx] : 0.0f;
+                qv[i][j] = in ? as_global(bd.q)[(size_t)gy * img.W + gx] : 0.0f;
             }
 
         const int tx = tid % DXLX, ty = tid / DXLX;
@@ -279,7 +328,7 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
                     const int r = r0 + 16 * i, c = c0 + 64 * j;
                     const int gy = y0 - R + r, gx = x0 - R + c;
                     const bool in = r < fh && c < fw && gy >= 0 && gy < img.H && gx >= 0 && gx < img.W;
-                    sv[i][j] = in ? img.map[(size_t)d * plane + (size_t)gy * img.W + gx] : 255;
+                    sv[i][j] = in ? as_global(img.map)[(size_t)d * plane + (size_t)gy * img.W + gx] : 255;
                 }
             for (int s = 0; s < S; ++s) {
                 int any = 0;
@@ -300,7 +349,7 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
                 correlate<DXP, false, 0>(lane_m, lp, adj.flip, d * S + s, dx);
             }
         }
-        float *__restrict__ out = bd.dx + (size_t)py * img.pitch + px0;
+        const auto out = as_global(bd.dx) + (size_t)py * img.pitch + px0;
 #pragma unroll
         for (int p = 0; p < DXP; ++p)
             if (py < img.H && px0 + p < img.W) out[p] = dx[p];
@@ -366,7 +415,7 @@ hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDe
     const int K = ks.D * ks.S, NW = adj.num_weights;
     if (ks.S > kMaxScales) return hipErrorInvalidValue;
     if (grad_lds_bytes(radius, K) > 65536 || dx_lds_bytes(radius, K) > 65536) return hipErrorInvalidValue;
-    const int rows = grid_prep * (kThreads / 64);
+    const int rows = grid_prep;
     const size_t row_floats = (size_t)rows * (NW + K);
     hipLaunchKernelGGL(reset_counter2, dim3(1), dim3(1), 0, stream, d_counter);
     hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(kThreads), 0, stream, partial, row_floats, 0.0f);

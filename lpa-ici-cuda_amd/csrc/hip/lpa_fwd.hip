@@ -116,12 +116,13 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
             den[p] = den[p] + binv[p];
         }
         if (map) {
+            const auto gmap = as_global(map) + (size_t)d * plane + pix;
 #pragma unroll
             for (int p = 0; p < P; ++p)
-                if (inside[p]) map[(size_t)d * plane + pix + p] = (uint8_t)sel[p];
+                if (inside[p]) gmap[p] = (uint8_t)sel[p];
         }
     }
-    float *__restrict__ yrow = y + (size_t)py * pitch + px0;
+    const auto yrow = as_global(y) + (size_t)py * pitch + px0;
 #pragma unroll
     for (int p = 0; p < P; ++p)
         if (inside[p]) yrow[p] = num[p] / den[p];

@@ -9,6 +9,15 @@
 
 namespace lpa {
 
+// Pointers read out of descriptors are generic; these casts let the compiler
+// use global (not flat) memory instructions, which do not tie up the LDS
+// wait counter.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *as_global(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // Separately rounded multiply and add: the same bits as the CPU path.
 __device__ __forceinline__ float mul_add_rounded(float acc, float a, float b)
 {
@@ -143,10 +152,11 @@ __device__ __forceinline__ void stage_tile(float *__restrict__ tile, int lp, con
 {
     const int tid = threadIdx.x;
     const int fw = TW + 2 * R, fh = TH + 2 * R;
+    const auto gx = as_global(x);
     for (int r = tid >> 6; r < fh; r += kThreads / 64) {
         const int gy = y0 - R + r;
         const bool row_in = gy >= 0 && gy < H;
-        const float *__restrict__ xrow = x + (size_t)(row_in ? gy : 0) * pitch;
+        const auto xrow = gx + (size_t)(row_in ? gy : 0) * pitch;
         for (int c = tid & 63; c < fw; c += 64) {
             const int gx = x0 - R + c;
             tile[r * lp + c] = (row_in && gx >= 0 && gx < W) ? xrow[gx] : 0.0f;
