@@ -572,7 +572,7 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     const size_t off_dxt = off_tiles + up(sizeof(uint2) * tiles.size());
     const size_t off_q = off_dxt + up(sizeof(uint2) * dx_tiles.size());
     const size_t off_part = off_q + up(sizeof(float) * qpx);
-    const size_t bytes = off_part + sizeof(float) * ((size_t)grid_prep * (NW + K) + K);
+    const size_t bytes = off_part + sizeof(float) * ((size_t)grid_prep * (NW + K) + 1 + 2 * (size_t)K);
     const size_t host_bytes = off_q;
     std::vector<unsigned char> host(host_bytes, 0);
     void *mem = nullptr;
@@ -604,7 +604,8 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     hipError_t e = hipMemcpyAsync(mem, host.data(), host_bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
     if (e == hipSuccess)
-        e = lpa::launch_backward(ks->dev(), ks->adjoint(), (const lpa::BwdDesc *)(base + off_desc),
+        e = lpa::launch_backward(ks->dev(), ks->has_rings ? ks->rings() : lpa::DevRings{nullptr, nullptr, nullptr},
+                                 ks->adjoint(), (const lpa::BwdDesc *)(base + off_desc),
                                  (const uint2 *)(base + off_tiles), (int)tiles.size(),
                                  (const uint2 *)(base + off_dxt), (int)dx_tiles.size(), (int *)base, ks->radius,
                                  grid_prep, grid_dx, (float *)(base + off_part), dkernel, s);

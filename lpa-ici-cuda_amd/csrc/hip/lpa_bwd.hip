@@ -137,18 +137,20 @@ __device__ __forceinline__ void reduce_scatter_add(const float (&v)[L], float *_
 // One run piece of L taps for the lane's GP pixels: yacc[p] += sum_j w[j] x,
 // and the wave's A partials for the piece's taps, sum_p m[p] x[p + j], added
 // into the wave's row at the piece's first run weight.
-template <int L>
+template <int L, bool YHAT>
 __device__ __forceinline__ void grad_piece(const float *__restrict__ src, const float *__restrict__ w,
                                            const float (&m)[GP], float (&yacc)[GP], float *__restrict__ row, int lane)
 {
     float v[L + GP - 1];
 #pragma unroll
     for (int q = 0; q < L + GP - 1; ++q) v[q] = src[q];
+    if constexpr (YHAT) {
 #pragma unroll
-    for (int j = 0; j < L; ++j) {
-        const float c = w[j];
+        for (int j = 0; j < L; ++j) {
+            const float c = w[j];
 #pragma unroll
-        for (int p = 0; p < GP; ++p) yacc[p] = __builtin_fmaf(c, v[j + p], yacc[p]);
+            for (int p = 0; p < GP; ++p) yacc[p] = __builtin_fmaf(c, v[j + p], yacc[p]);
+        }
     }
     float part[L];
 #pragma unroll
@@ -163,12 +165,17 @@ __device__ __forceinline__ void grad_piece(const float *__restrict__ src, const 
 
 // Pass 1.  For every tile: q = g / sum_d inv_d (written to the q map for pass
 // 2); then, per direction and for every scale selected by some pixel of the
-// wave, one walk over that kernel's runs recomputes yhat for the pixels that
-// selected it and forms the wave's A partials with the masked coefficients
-// m_p = [s_d(p) = s] q_p inv, and the wave's E partial.  Each work-group
+// wave, one walk over that kernel's runs forms the wave's A partials with the
+// masked coefficients m_p = [s_d(p) = s] q_p inv, and the wave's E partial.
+// yhat is recomputed with the forward's own arithmetic, so it equals the
+// estimate y was built from bit for bit (E is a difference yhat - y and
+// enters dL/dw beside A, where the two can cancel): RING walks the ring sums
+// up to the wave's deepest selected scale; otherwise the same weighted walk
+// that forms A also accumulates yhat.  Each work-group
 // owns one row of `partial` ([grid][NW + K]); its waves add into it with
 // float atomics that return nothing.  No barrier after the tile is staged.
-__global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoint adj,
+template <bool RING>
+__global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                                                             const BwdDesc *__restrict__ imgs,
                                                             const uint2 *__restrict__ tiles, int num_tiles,
                                                             int *__restrict__ counter, int R,
@@ -225,6 +232,27 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoi
             const auto gmd = gmap + (size_t)d * plane;
 #pragma unroll
             for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? gmd[p] : 255;
+            float yh[GP];
+#pragma unroll
+            for (int p = 0; p < GP; ++p) yh[p] = 0.0f;
+            if constexpr (RING) {
+                // The forward's ring walk: running sum over rings 0..s, yhat = w_s * sum.
+                float rs[GP];
+#pragma unroll
+                for (int p = 0; p < GP; ++p) rs[p] = 0.0f;
+                for (int s = 0; s < S; ++s) {
+                    bool need = false;
+#pragma unroll
+                    for (int p = 0; p < GP; ++p) need |= inside[p] && sd[p] >= s;
+                    if (!__any(need)) break;
+                    const int k = d * S + s;
+                    if (need) correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr, rings.kern[k], rs);
+                    const float w = rings.scale[k];
+#pragma unroll
+                    for (int p = 0; p < GP; ++p)
+                        if (sd[p] == s) yh[p] = rs[p] * w;
+                }
+            }
             for (int s = 0; s < S; ++s) {
                 bool want = false;
 #pragma unroll
@@ -248,7 +276,7 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoi
                         float *__restrict__ rrow = row + run.w + j0;
                         switch (min(kGradPiece, run.z - j0)) {
 #define LPA_GRAD_CASE(L)                                                                                           \
-    case L: grad_piece<L>(src, w, m, yacc, rrow, lane); break;
+    case L: grad_piece<L, !RING>(src, w, m, yacc, rrow, lane); break;
                             LPA_GRAD_CASE(1) LPA_GRAD_CASE(2) LPA_GRAD_CASE(3) LPA_GRAD_CASE(4) LPA_GRAD_CASE(5)
                             LPA_GRAD_CASE(6) LPA_GRAD_CASE(7) LPA_GRAD_CASE(8)
 #undef LPA_GRAD_CASE
@@ -257,9 +285,14 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevAdjoi
                     }
                 }
                 // E_k: sum over the pixels that selected s of c (yhat - y).
+                if constexpr (!RING) {
+#pragma unroll
+                    for (int p = 0; p < GP; ++p)
+                        if (sd[p] == s) yh[p] = yacc[p];
+                }
                 float e = 0.0f;
 #pragma unroll
-                for (int p = 0; p < GP; ++p) e = __builtin_fmaf(m[p], yacc[p] - yv[p], e);
+                for (int p = 0; p < GP; ++p) e = __builtin_fmaf(m[p], yh[p] - yv[p], e);
                 e = wave_total(e);
                 if (lane == 0) unsafeAtomicAdd(row + NW + k, e);
             }
@@ -357,20 +390,24 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
     }
 }
 
-// Pass 3a: E_k = sum of the rows' E entries.  One thread per kernel.
+// Pass 3a: E_k = sum of the rows' E entries.  One thread per kernel.  The
+// rows are summed in double: dL/dw = A - 2 w E / sumsq is a difference of
+// two terms that nearly cancel for some taps, so both are carried to the
+// final subtraction with no more rounding than the per-row sums hold.
 __global__ __launch_bounds__(kThreads) void reduce_e_kernel(const float *__restrict__ partial, int rows, int NW,
-                                                            int K, float *__restrict__ e_out)
+                                                            int K, double *__restrict__ e_out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
-    float s = 0.0f;
-    for (int b = 0; b < rows; ++b) s += partial[(size_t)b * (NW + K) + NW + k];
+    double s = 0.0;
+    for (int b = 0; b < rows; ++b) s += (double)partial[(size_t)b * (NW + K) + NW + k];
     e_out[k] = s;
 }
 
-// Pass 3b: dkernel[i] += sum_b A_b[i] - 2 w[i] E_k / sumsq_k.
+// Pass 3b: dkernel[i] += sum_b A_b[i] - 2 w[i] E_k / sumsq_k, in double until
+// the one rounding to float at the end.
 __global__ __launch_bounds__(kThreads) void reduce_a_kernel(const float *__restrict__ partial, int rows, int NW,
-                                                            int K, const float *__restrict__ e_k,
+                                                            int K, const double *__restrict__ e_k,
                                                             const float *__restrict__ weights,
                                                             const int *__restrict__ tap_kernel,
                                                             const float *__restrict__ sumsq,
@@ -378,10 +415,10 @@ __global__ __launch_bounds__(kThreads) void reduce_a_kernel(const float *__restr
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NW) return;
-    float s = 0.0f;
-    for (int b = 0; b < rows; ++b) s += partial[(size_t)b * (NW + K) + i];
+    double s = 0.0;
+    for (int b = 0; b < rows; ++b) s += (double)partial[(size_t)b * (NW + K) + i];
     const int k = tap_kernel[i];
-    dkernel[i] += s - 2.0f * weights[i] * e_k[k] / sumsq[k];
+    dkernel[i] = (float)((double)dkernel[i] + (s - 2.0 * (double)weights[i] * e_k[k] / (double)sumsq[k]));
 }
 
 size_t grad_lds_bytes(int R, int K) { return sizeof(float) * ((size_t)grad_tab_offset(R) + 3 * K); }
@@ -397,7 +434,7 @@ void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, 
 {
     int b = 0;
     (void)num_scales;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_grad_kernel, kThreads,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_grad_kernel<false>, kThreads,
                                                      grad_lds_bytes(radius, num_kernels)) != hipSuccess)
         b = 1;
     *prep = b > 0 ? b : 1;
@@ -408,7 +445,8 @@ void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, 
     *dx = b > 0 ? b : 1;
 }
 
-hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDesc *d_imgs, const uint2 *d_tiles,
+hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAdjoint &adj, const BwdDesc *d_imgs,
+                           const uint2 *d_tiles,
                            int num_tiles, const uint2 *dx_tiles, int num_dx_tiles, int *d_counter, int radius,
                            int grid_prep, int grid_dx, float *partial, float *dkernel, hipStream_t stream)
 {
@@ -419,12 +457,17 @@ hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDe
     const size_t row_floats = (size_t)rows * (NW + K);
     hipLaunchKernelGGL(reset_counter2, dim3(1), dim3(1), 0, stream, d_counter);
     hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(kThreads), 0, stream, partial, row_floats, 0.0f);
-    hipLaunchKernelGGL(bwd_grad_kernel, dim3(grid_prep), dim3(kThreads), grad_lds_bytes(radius, K), stream, ks, adj,
-                       d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
+    if (rings.runs)
+        hipLaunchKernelGGL(bwd_grad_kernel<true>, dim3(grid_prep), dim3(kThreads), grad_lds_bytes(radius, K), stream,
+                           ks, rings, adj, d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
+    else
+        hipLaunchKernelGGL(bwd_grad_kernel<false>, dim3(grid_prep), dim3(kThreads), grad_lds_bytes(radius, K), stream,
+                           ks, rings, adj, d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
     hipLaunchKernelGGL(bwd_dx_kernel, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, adj, d_imgs,
                        dx_tiles, num_dx_tiles, d_counter, radius);
     if (dkernel) {
-        float *e_k = partial + row_floats;  // K floats after the rows
+        // K doubles after the rows, at the next 8-byte boundary
+        double *e_k = (double *)(partial + ((row_floats + 1) & ~(size_t)1));
         hipLaunchKernelGGL(reduce_e_kernel, dim3((K + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, partial,
                            rows, NW, K, e_k);
         hipLaunchKernelGGL(reduce_a_kernel, dim3((NW + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, partial,

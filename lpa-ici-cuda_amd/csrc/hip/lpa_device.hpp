@@ -112,11 +112,14 @@ int blocks_per_cu(int shape, int radius);
 // Adjoint (lpa_bwd.hip).  The first pass walks 64 x 32 tiles (d_tiles), the
 // dx pass 64 x 64 tiles (dx_tiles); both queues are laid out as for the
 // batched forward ({image, (tile_y << 16) | tile_x}).
-//   partial: [grid_prep][num_weights + K] floats (+K for the E sums), one row
+//   partial: [grid_prep][num_weights + K] floats (+1+2K for the E sums in double), one row
 //            per work-group of the first pass, accumulated by its waves
 //   dkernel: += reduction of the rows plus the variance term (may be null)
 constexpr int kBwdTileW = 64, kBwdTileH = 32, kDxTile = 64;
-hipError_t launch_backward(const DevKset &ks, const DevAdjoint &adj, const BwdDesc *d_imgs, const uint2 *d_tiles,
+//   rings:   the ring form (runs == nullptr when the set has none); the
+//            forward's default walk, which the adjoint repeats for yhat
+hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAdjoint &adj, const BwdDesc *d_imgs,
+                           const uint2 *d_tiles,
                            int num_tiles, const uint2 *dx_tiles, int num_dx_tiles, int *d_counter, int radius,
                            int grid_prep, int grid_dx, float *partial, float *dkernel, hipStream_t stream);
 // Work-groups per CU of the two tile passes.
