@@ -48,32 +48,37 @@ struct ImgDesc {
 };
 
 // Compiled tile shapes: output tile tw x th pixels, px adjacent pixels per
-// lane along a row, 256 lanes per work-group, and the inner-loop body
-// (1: length-dispatched unrolled runs, 0: sliding-window loop).
+// lane along a row, 256 lanes per work-group, the inner-loop body
+// (1: length-dispatched unrolled runs, 0: sliding-window loop) and the LDS
+// row pitch's residue mod 32 (the bank offset between consecutive rows).
 struct TileShape {
-    int tw, th, px, body;
+    int tw, th, px, body, pm;
 };
 constexpr int kThreads = 256;
-constexpr int kNumShapes = 6;
+constexpr int kNumShapes = 7;
 constexpr TileShape kShapes[kNumShapes] = {
-    {32, 32, 4, 1},  // 0: wave covers 32x8 px
-    {32, 32, 4, 0},  // 1: same, window loop
-    {64, 16, 4, 1},  // 2: wave covers 64x4 px
-    {32, 16, 2, 1},  // 3: wave covers 32x4 px
-    {64, 32, 8, 1},  // 4: wave covers 64x8 px
-    {32, 64, 8, 1},  // 5: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
+    {32, 32, 4, 1, 1},  // 0: wave covers 32x8 px
+    {32, 32, 4, 0, 1},  // 1: same, window loop
+    {64, 16, 4, 1, 1},  // 2: wave covers 64x4 px
+    {32, 16, 2, 1, 1},  // 3: wave covers 32x4 px
+    {64, 32, 8, 1, 1},  // 4: wave covers 64x8 px
+    {32, 64, 8, 1, 1},  // 5: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
+    {32, 64, 8, 1, 2},  // 6: as 5, rows two banks apart: the dword pairs of 16 lanes (4 rows) are disjoint
 };
 constexpr int kMaxRadius = 31;
 
 // LDS row pitch for a tile of width tw and halo radius r: at least tw + 2r and
-// congruent to 1 mod 32, so lanes four pixels apart on consecutive rows fall
-// in different banks.
-__host__ __device__ inline int lds_pitch(int tw, int r)
+// congruent to pm mod 32 (default 1: lanes four pixels apart on consecutive
+// rows fall in different banks).
+__host__ __device__ inline int lds_pitch(int tw, int r, int pm = 1)
 {
     int p = tw + 2 * r;
-    return p + ((1 - p) % 32 + 32) % 32;
+    return p + ((pm - p) % 32 + 32) % 32;
 }
-__host__ __device__ inline int lds_floats(int tw, int th, int r) { return (th + 2 * r) * lds_pitch(tw, r); }
+__host__ __device__ inline int lds_floats(int tw, int th, int r, int pm = 1)
+{
+    return (th + 2 * r) * lds_pitch(tw, r, pm);
+}
 
 // One image of an adjoint batch: the forward's image (x, y = the forward's
 // output, map = its scale indices, tab) plus the upstream gradient g, the

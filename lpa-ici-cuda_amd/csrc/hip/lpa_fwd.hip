@@ -25,7 +25,7 @@ namespace {
 // One output tile at (x0, y0) of an image.  `tile` is the work-group's LDS
 // buffer of lds_floats(TW, TH, R) floats.  Contains one barrier, reached by
 // every lane.
-template <int TW, int TH, int P, int MODE, int BODY>
+template <int TW, int TH, int P, int MODE, int BODY, int PM>
 __device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks, const DevRings &rings,
                                              const float *__restrict__ tab, const float *__restrict__ x,
                                              float *__restrict__ y, uint8_t *__restrict__ map, int W, int H,
@@ -34,7 +34,7 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
     constexpr bool EXACT = MODE == kWalkExact;
     constexpr int LX = TW / P;
     static_assert(LX * TH == kThreads, "tile shape must use 256 lanes");
-    const int lp = lds_pitch(TW, R);
+    const int lp = lds_pitch(TW, R, PM);
     const int tid = threadIdx.x;
 
     // Stage tile + halo; pixels outside the image read as zero, which is the
@@ -75,12 +75,13 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
             binv[p] = 0.0f;
             sel[p] = 0;
         }
-        for (int s = 0; s < S; ++s) {
-            bool alive = false;
+        // alive: some pixel of the lane still has a non-empty interval (it
+        // is the `ok` of the previous scale; an empty intersection stays
+        // empty).  The wave stops walking this direction once no lane is.
+        bool alive = false;
 #pragma unroll
-            for (int p = 0; p < P; ++p) alive |= hi[p] >= lo[p];
-            // The wave stops walking this direction once no lane has a
-            // non-empty interval left (an empty intersection stays empty).
+        for (int p = 0; p < P; ++p) alive |= inside[p];
+        for (int s = 0; s < S; ++s) {
             if (!__any(alive)) break;
             const int k = d * S + s;
             float acc[P];
@@ -97,18 +98,22 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
                 if (alive) correlate<P, EXACT, BODY>(lane, lp, ks, k, acc);
             }
             const float thr = thr_t[k], inv = inv_t[k];
+            bool any_ok = false;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                const float l = fmaxf(lo[p], acc[p] - thr);
-                const float h = fminf(hi[p], acc[p] + thr);
+                const float l = max_f32(lo[p], acc[p] - thr);
+                const float h = min_f32(hi[p], acc[p] + thr);
                 lo[p] = l;
                 hi[p] = h;
-                if (h >= l) {
+                const bool ok = h >= l;
+                if (ok) {
                     best[p] = acc[p];
                     binv[p] = inv;
                     sel[p] = s;
                 }
+                any_ok |= ok;
             }
+            alive = any_ok;
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
@@ -141,7 +146,7 @@ __global__ __launch_bounds__(kThreads) void fwd_plain_kernel(DevKset ks, DevRing
     const int t = (b & 7) * per_xcd + (b >> 3);
     if (t >= num_tiles) return;  // whole work-group, before any barrier
     const int tx = t % img.tiles_x, ty = t / img.tiles_x;
-    process_tile<sh.tw, sh.th, sh.px, MODE, sh.body>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W, img.H,
+    process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W, img.H,
                                                     img.pitch, tx * sh.tw, ty * sh.th, R);
 }
 
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, DevRi
         if (t >= num_tiles) break;
         const uint2 e = tiles[t];
         const ImgDesc img = imgs[e.x];
-        process_tile<sh.tw, sh.th, sh.px, MODE, sh.body>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W,
+        process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W,
                                                         img.H, img.pitch, (int)(e.y & 0xffffu) * sh.tw,
                                                         (int)(e.y >> 16) * sh.th, R);
         __syncthreads();  // the LDS tile and `next` are reused
@@ -180,7 +185,7 @@ hipError_t plain_launch(const DevKset &ks, const DevRings &rings, const ImgDesc 
     const int tiles_y = (img.H + sh.th - 1) / sh.th;
     const int num_tiles = img.tiles_x * tiles_y;
     const int per_xcd = (num_tiles + 7) / 8;
-    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R);
+    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R, sh.pm);
     hipLaunchKernelGGL((fwd_plain_kernel<SH, MODE>), dim3(8 * per_xcd), dim3(kThreads), lds, stream, ks, rings, img,
                        R, num_tiles, per_xcd);
     return hipGetLastError();
@@ -202,7 +207,7 @@ hipError_t batched_launch(const DevKset &ks, const DevRings &rings, const ImgDes
                           int num_tiles, int *counter, int R, int grid, hipStream_t stream)
 {
     constexpr TileShape sh = kShapes[SH];
-    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R);
+    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R, sh.pm);
     hipLaunchKernelGGL((fwd_batched_kernel<SH, MODE>), dim3(grid), dim3(kThreads), lds, stream, ks, rings, imgs, tiles,
                        num_tiles, counter, R);
     return hipGetLastError();
@@ -223,7 +228,7 @@ hipError_t batched_dispatch(WalkMode mode, const DevKset &ks, const DevRings &ri
 template <int SH>
 int occupancy(int R)
 {
-    const size_t lds = sizeof(float) * (size_t)lds_floats(kShapes[SH].tw, kShapes[SH].th, R);
+    const size_t lds = sizeof(float) * (size_t)lds_floats(kShapes[SH].tw, kShapes[SH].th, R, kShapes[SH].pm);
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, kWalkFma>, kThreads, lds) !=
         hipSuccess)
@@ -243,6 +248,7 @@ hipError_t launch_forward_plain(int shape, WalkMode mode, const DevKset &ks, con
     case 3: return plain_dispatch<3>(mode, ks, rings, img, radius, stream);
     case 4: return plain_dispatch<4>(mode, ks, rings, img, radius, stream);
     case 5: return plain_dispatch<5>(mode, ks, rings, img, radius, stream);
+    case 6: return plain_dispatch<6>(mode, ks, rings, img, radius, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -254,7 +260,7 @@ hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, c
     switch (shape) {
 #define LPA_B(SH)                                                                                                  \
     case SH: return batched_dispatch<SH>(mode, ks, rings, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-        LPA_B(0) LPA_B(1) LPA_B(2) LPA_B(3) LPA_B(4) LPA_B(5)
+        LPA_B(0) LPA_B(1) LPA_B(2) LPA_B(3) LPA_B(4) LPA_B(5) LPA_B(6)
 #undef LPA_B
     default: return hipErrorInvalidValue;
     }
@@ -269,6 +275,7 @@ int blocks_per_cu(int shape, int radius)
     case 3: return occupancy<3>(radius);
     case 4: return occupancy<4>(radius);
     case 5: return occupancy<5>(radius);
+    case 6: return occupancy<6>(radius);
     default: return 1;
     }
 }

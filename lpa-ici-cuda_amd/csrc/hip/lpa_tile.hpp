@@ -25,6 +25,23 @@ __device__ __forceinline__ float mul_add_rounded(float acc, float a, float b)
     return acc + a * b;
 }
 
+// max(a, b) and min(a, b) as single VALU instructions.  fmaxf/fminf make the
+// compiler quiet signalling NaNs on loop-carried operands first (one extra
+// v_max per operand); the values here are arithmetic results, never
+// signalling NaNs, so the plain instruction gives the same result.
+__device__ __forceinline__ float max_f32(float a, float b)
+{
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float min_f32(float a, float b)
+{
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 template <bool EXACT>
 __device__ __forceinline__ float tap(float acc, float w, float v)
 {
