@@ -62,13 +62,27 @@ def make_image(w, h, seed, sigma, noise_seed):
     return lpa_ici.make_noisy(lpa_ici.synth_image(w, h, seed), sigma, noise_seed)
 
 
-def executed_taps(ks, maps, ring):
-    """Taps the hardware adds: the walk's nonzero taps, or for the ring form
-    only the ring taps of scales 0..min(s*+1, S-1)."""
-    if not ring:
-        return useful_taps(maps, ks.runs()["nnz"], ks.S)
-    r = ks.ring_taps_per_kernel()
-    return useful_taps(maps, r, ks.S)
+# VALU lane-operations per second without packed math: 256 CUs x 2.4 GHz x
+# 4 SIMDs x 16 lanes (one 32-bit add or FMA per lane per clock).
+MI355X_LANE_OPS = 256 * 2.4e9 * 64
+
+
+def walk_of(ks, flags):
+    if ks.info()["ring_form"] != 1 or flags & (lpa_ici.FWD_WEIGHTED | lpa_ici.FWD_EXACT):
+        return "weighted FMA"
+    return "float ring sums" if flags & lpa_ici.FWD_RING_FLOAT else "fixed-point ring sums"
+
+
+def executed_ops(ks, maps, walk):
+    """Arithmetic the hardware does per tap or run, counted from the scale
+    maps (scales 0..min(s*+1, S-1) of every pixel and direction): one FMA per
+    nonzero tap (weighted), one add per ring tap (float ring sums), two
+    integer adds per ring run (fixed point)."""
+    if walk == "weighted FMA":
+        return useful_taps(maps, ks.runs()["nnz"], ks.S), "f32 fma"
+    if walk == "float ring sums":
+        return useful_taps(maps, ks.ring_taps_per_kernel(), ks.S), "f32 add"
+    return 2 * useful_taps(maps, ks.ring_runs_per_kernel(), ks.S), "i32 add"
 
 
 def plain_leg(ks, x_host, steps, warmup, stream, label, flags=0):
@@ -79,19 +93,18 @@ def plain_leg(ks, x_host, steps, warmup, stream, label, flags=0):
     with torch.cuda.stream(stream):
         ms = timed(lambda: plan.run(stream), steps, warmup, stream)
     taps = useful_taps(maps, ks.runs()["nnz"], ks.S)
-    ring = ks.info()["ring_form"] == 1 and not (flags & lpa_ici.FWD_WEIGHTED)
-    ex = executed_taps(ks, maps, ring)
+    walk = walk_of(ks, flags)
+    ex, op = executed_ops(ks, maps, walk)
     npx = x.numel()
     tflops = 2.0 * taps / (ms * 1e-3) / 1e12
-    # executed: one add (ring sums) or one FMA (weighted) per executed tap
-    ex_tflops = (1.0 if ring else 2.0) * ex / (ms * 1e-3) / 1e12
+    ex_rate = ex / (ms * 1e-3)
     log(f"[{label}] {x.shape[1]}x{x.shape[0]}: {ms:.3f} ms/pass, {npx / ms / 1e3:.1f} Mpx/s, "
-        f"{taps / npx:.0f} useful taps/px ({tflops:.2f} TFLOP/s algorithmic), {ex / npx:.0f} executed taps/px "
-        f"({ex_tflops:.2f} TFLOP/s executed, {'ring sums' if ring else 'weighted FMA'})")
+        f"{taps / npx:.0f} useful taps/px ({tflops:.2f} TFLOP/s algorithmic), {walk}: {ex / npx:.0f} {op}/px "
+        f"executed ({ex_rate / MI355X_LANE_OPS:.3f} of the VALU lane-op rate)")
     plan.close()
     return {"ms": ms, "mpx_s": npx / ms / 1e3, "useful_taps": taps, "taps_per_px": taps / npx, "tflops": tflops,
-            "executed_taps_per_px": ex / npx, "executed_tflops": ex_tflops,
-            "walk": "ring sums" if ring else "weighted FMA", "y": y, "x": x}
+            "executed_op": op, "executed_ops_per_px": ex / npx, "executed_tops": ex_rate / 1e12,
+            "executed_frac_of_lane_ops": ex_rate / MI355X_LANE_OPS, "walk": walk, "y": y, "x": x}
 
 
 def batched_leg(ks, steps, warmup, stream, n_images=256, seed=3):
@@ -164,7 +177,7 @@ def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
 def measured_traffic(walk):
     """HBM bytes per launch of the 4096^2 forward from the committed rocprofv3
     counter runs (profiles/forward_plain_4096_symmetric.md), or None."""
-    key = {"ring sums": "ring_sums", "weighted FMA": "weighted_fma"}[walk]
+    key = {"fixed-point ring sums": "ring_fixed", "float ring sums": "ring_sums", "weighted FMA": "weighted_fma"}[walk]
     try:
         for line in open(os.path.join(REPO, "profiles", "forward_plain_4096_symmetric.traffic")):
             if line.startswith(key + ":"):
@@ -334,16 +347,18 @@ def main():
                      "algorithmic_flop_per_launch": 2 * main_leg["useful_taps"],
                      "useful_taps_per_px": round(main_leg["taps_per_px"], 1),
                      "walk": main_leg["walk"],
-                     "executed_taps_per_px": round(main_leg["executed_taps_per_px"], 1),
-                     "executed_tflops": round(main_leg["executed_tflops"], 3),
-                     "executed_frac": round(main_leg["executed_tflops"] / MI355X_FP32_PEAK_TFLOPS, 4)},
+                     "executed_op": main_leg["executed_op"],
+                     "executed_ops_per_px": round(main_leg["executed_ops_per_px"], 1),
+                     "executed_tops": round(main_leg["executed_tops"], 3),
+                     "executed_frac_of_lane_ops": round(main_leg["executed_frac_of_lane_ops"], 4)},
     }
-    if main_leg["walk"] == "ring sums":
-        w = plain_leg(ks, x_host, args.steps, args.warmup, stream, "plain symmetric weighted",
-                      flags=lpa_ici.FWD_WEIGHTED)
-        rec["weighted_walk"] = {"mpx_s": round(w["mpx_s"], 2), "ms": round(w["ms"], 4),
-                                "tflops": round(w["tflops"], 3),
-                                "frac": round(w["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4)}
+    if main_leg["walk"] == "fixed-point ring sums":
+        for key, flags in (("float_ring_walk", lpa_ici.FWD_RING_FLOAT), ("weighted_walk", lpa_ici.FWD_WEIGHTED)):
+            w = plain_leg(ks, x_host, args.steps, args.warmup, stream, f"plain symmetric {key}", flags=flags)
+            rec[key] = {"mpx_s": round(w["mpx_s"], 2), "ms": round(w["ms"], 4), "tflops": round(w["tflops"], 3),
+                        "frac": round(w["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4),
+                        "executed_op": w["executed_op"], "executed_ops_per_px": round(w["executed_ops_per_px"], 1),
+                        "executed_frac_of_lane_ops": round(w["executed_frac_of_lane_ops"], 4)}
     if not args.skip_cpu:
         rec["cpu_baseline"] = cpu_baseline(x_host)
         log(f"[cpu] {rec['cpu_baseline']}")

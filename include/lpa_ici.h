@@ -73,6 +73,12 @@ typedef struct lpa_kset_info {
                           contains the previous one's, so the fast walk builds each
                           scale's estimate from the previous sum plus the new ring */
     int num_ring_taps; /* taps over all rings (ring form only) */
+    int ring_frac_bits; /* ring form: the default walk sums tile values in fixed
+                           point, q(v) = rint(v * 2^ring_frac_bits), exactly in
+                           32-bit integers; yhat = (float)sum * (w * 2^-F) */
+    float ring_range;   /* ring form: a tile window whose values all lie strictly
+                           within +-ring_range is walked in fixed point; any
+                           other window (or a non-finite value) in float */
 } lpa_kset_info;
 lpa_status lpa_kset_get_info(lpa_kset_t ks, lpa_kset_info* info);
 
@@ -107,7 +113,9 @@ enum {
     LPA_FWD_EXACT = 2u,  /* separately rounded mul and add in the reference's tap
                             order: the same bits as the CPU v0_0 path (slower) */
     LPA_FWD_GRAPH = 4u,  /* plans only: capture the launch into a hipGraph and replay it */
-    LPA_FWD_WEIGHTED = 8u /* use the weighted run walk even for a ring-form kernel set */
+    LPA_FWD_WEIGHTED = 8u, /* use the weighted run walk even for a ring-form kernel set */
+    LPA_FWD_RING_FLOAT = 16u /* ring-form sets: float ring sums for every window
+                                (the default takes fixed point where it fits) */
 };
 
 /* Denoise n images.  Without LPA_FWD_PLAIN all n go through one persistent

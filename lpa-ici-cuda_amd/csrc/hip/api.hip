@@ -48,13 +48,23 @@ struct lpa_kset {
     std::vector<int2> ring_kern;
     std::vector<float> ring_scale;
     int ring_taps = 0;
+    int ring_frac_bits = 0;      // F: ring sums take tile values as rint(v * 2^F)
+    float ring_range = 0.0f;     // fixed point for windows with every |v| below this
     int4 *d_ring_runs = nullptr;
     int2 *d_ring_kern = nullptr;
     float *d_ring_scale = nullptr;
     bool uploaded = false;
     std::mutex upload_mutex;
     lpa::DevKset dev() const { return lpa::DevKset{d_runs, d_weights, d_kern, D, S}; }
-    lpa::DevRings rings() const { return lpa::DevRings{d_ring_runs, d_ring_kern, d_ring_scale}; }
+    lpa::DevRings rings() const
+    {
+        return lpa::DevRings{d_ring_runs,
+                             d_ring_kern,
+                             d_ring_scale,
+                             std::ldexp(1.0f, ring_frac_bits),
+                             std::ldexp(1.0f, -ring_frac_bits),
+                             ring_range};
+    }
     lpa::DevAdjoint adjoint() const
     {
         return lpa::DevAdjoint{lpa::DevKset{d_runs_flip, d_weights_flip, d_kern, D, S},
@@ -202,6 +212,17 @@ void build_rings(lpa_kset *ks, const float *dense, const int *offsets, const int
             ks->ring_kern.push_back(int2{first_run, (int)ks->ring_runs.size() - first_run});
             ks->ring_taps += (int)ring.size();
         }
+    // Fixed point for ring sums: with n_max the largest support, F is the
+    // largest power making n_max * 2 * 2^F < 2^31, and the range bound r
+    // keeps every sum exact: |rint(v 2^F)| < r 2^F + 1/2 for |v| < r, so a
+    // support sum stays below n_max (r 2^F + 1/2) < 2^31.
+    size_t nmax = 1;
+    for (const auto &sp : support) nmax = std::max(nmax, sp.size());
+    const double lim = 2147483647.0;
+    int F = (int)std::floor(std::log2(lim / (2.0 * (double)nmax)));
+    F = std::max(0, std::min(F, 30));
+    ks->ring_frac_bits = F;
+    ks->ring_range = (float)(lim * (1.0 - 1.0 / 4096.0) / ((double)nmax * std::ldexp(1.0, F)));
     ks->has_rings = true;
 }
 
@@ -387,7 +408,8 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
 lpa::WalkMode walk_mode(const lpa_kset *ks, unsigned flags)
 {
     if (flags & LPA_FWD_EXACT) return lpa::kWalkExact;
-    if (ks->has_rings && !(flags & LPA_FWD_WEIGHTED)) return lpa::kWalkRing;
+    if (ks->has_rings && !(flags & LPA_FWD_WEIGHTED))
+        return (flags & LPA_FWD_RING_FLOAT) ? lpa::kWalkRingFloat : lpa::kWalkRing;
     return lpa::kWalkFma;
 }
 
@@ -466,6 +488,8 @@ lpa_status lpa_kset_get_info(lpa_kset_t ks, lpa_kset_info *info)
     info->max_radius = ks->radius;
     info->ring_form = ks->has_rings ? 1 : 0;
     info->num_ring_taps = ks->ring_taps;
+    info->ring_frac_bits = ks->ring_frac_bits;
+    info->ring_range = ks->ring_range;
     return LPA_OK;
 }
 
@@ -604,7 +628,7 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     hipError_t e = hipMemcpyAsync(mem, host.data(), host_bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
     if (e == hipSuccess)
-        e = lpa::launch_backward(ks->dev(), ks->has_rings ? ks->rings() : lpa::DevRings{nullptr, nullptr, nullptr},
+        e = lpa::launch_backward(ks->dev(), ks->has_rings ? ks->rings() : lpa::DevRings{},
                                  ks->adjoint(), (const lpa::BwdDesc *)(base + off_desc),
                                  (const uint2 *)(base + off_tiles), (int)tiles.size(),
                                  (const uint2 *)(base + off_dxt), (int)dx_tiles.size(), (int *)base, ks->radius,

@@ -170,8 +170,11 @@ __device__ __forceinline__ void grad_piece(const float *__restrict__ src, const 
 // yhat is recomputed with the forward's own arithmetic, so it equals the
 // estimate y was built from bit for bit (E is a difference yhat - y and
 // enters dL/dw beside A, where the two can cancel): RING walks the ring sums
-// up to the wave's deepest selected scale; otherwise the same weighted walk
-// that forms A also accumulates yhat.  Each work-group
+// up to the wave's deepest selected scale, in fixed point when every value of
+// this window is in range (the forward decides the same per window of its own
+// tile, so the two agree unless an out-of-range value lies within reach of
+// one window and not the other); otherwise the same weighted walk that forms
+// A also accumulates yhat.  Each work-group
 // owns one row of `partial` ([grid][NW + K]); its waves add into it with
 // float atomics that return nothing.  No barrier after the tile is staged.
 template <bool RING>
@@ -201,8 +204,15 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
         const ImgDesc &img = bd.img;
         const int x0 = (int)(te.y & 0xffffu) * GW, y0 = (int)(te.y >> 16) * GH;
         for (int i = tid; i < 3 * K; i += kThreads) tab[i] = img.tab[i];
-        stage_tile<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
-        __syncthreads();
+        // fixed: the forward's fixed-point ring sums apply to this window
+        // (every value within the range); the test is also the barrier.
+        bool fixed = false;
+        if constexpr (RING) {
+            fixed = stage_tile_fits<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R, rings.qrange);
+        } else {
+            stage_tile<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
+            __syncthreads();
+        }
 
         const float *inv_t = tab + 2 * K;
         const int tx = tid % GLX, ty = tid / GLX;
@@ -236,21 +246,36 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
 #pragma unroll
             for (int p = 0; p < GP; ++p) yh[p] = 0.0f;
             if constexpr (RING) {
-                // The forward's ring walk: running sum over rings 0..s, yhat = w_s * sum.
+                // The forward's ring walk: running sum over rings 0..s up to
+                // the deepest selected scale; yhat = (w_s 2^-F) * (fixed-point
+                // sum), or w_s * (float sum) for a window outside the range.
                 float rs[GP];
+                uint32_t rq[GP];
 #pragma unroll
-                for (int p = 0; p < GP; ++p) rs[p] = 0.0f;
+                for (int p = 0; p < GP; ++p) {
+                    rs[p] = 0.0f;
+                    rq[p] = 0u;
+                }
                 for (int s = 0; s < S; ++s) {
                     bool need = false;
 #pragma unroll
                     for (int p = 0; p < GP; ++p) need |= inside[p] && sd[p] >= s;
                     if (!__any(need)) break;
                     const int k = d * S + s;
-                    if (need) correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr, rings.kern[k], rs);
-                    const float w = rings.scale[k];
+                    if (fixed) {
+                        if (need) quantized_runs<GP>(lane_x, lp, rings.runs, rings.kern[k], rings.qscale, rq);
+                        const float wq = rings.scale[k] * rings.qinv;
 #pragma unroll
-                    for (int p = 0; p < GP; ++p)
-                        if (sd[p] == s) yh[p] = rs[p] * w;
+                        for (int p = 0; p < GP; ++p)
+                            if (sd[p] == s) yh[p] = mul_rounded((float)(int)rq[p], wq);
+                    } else {
+                        if (need)
+                            correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr, rings.kern[k], rs);
+                        const float w = rings.scale[k];
+#pragma unroll
+                        for (int p = 0; p < GP; ++p)
+                            if (sd[p] == s) yh[p] = rs[p] * w;
+                    }
                 }
             }
             for (int s = 0; s < S; ++s) {

@@ -32,10 +32,15 @@ struct DevRings {
     const int4 *runs;
     const int2 *kern;
     const float *scale;
+    float qscale;  // 2^F: fixed-point scale of tile values for ring sums
+    float qinv;    // 2^-F
+    float qrange;  // windows whose values all lie strictly within +-qrange use fixed point
 };
 
-// Arithmetic of the forward walk.
-enum WalkMode { kWalkFma = 0, kWalkExact = 1, kWalkRing = 2 };
+// Arithmetic of the forward walk: FMA over the weighted runs, the
+// reference's separately rounded mul/add, ring sums in fixed point (float
+// ring sums for a window outside the fixed-point range), float ring sums.
+enum WalkMode { kWalkFma = 0, kWalkExact = 1, kWalkRing = 2, kWalkRingFloat = 3 };
 
 // One image of a batch.  tab points at its [3][K] table.
 struct ImgDesc {

@@ -22,29 +22,16 @@
 namespace lpa {
 namespace {
 
-// One output tile at (x0, y0) of an image.  `tile` is the work-group's LDS
-// buffer of lds_floats(TW, TH, R) floats.  Contains one barrier, reached by
-// every lane.
-template <int TW, int TH, int P, int MODE, int BODY, int PM>
-__device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks, const DevRings &rings,
-                                             const float *__restrict__ tab, const float *__restrict__ x,
-                                             float *__restrict__ y, uint8_t *__restrict__ map, int W, int H,
-                                             int pitch, int x0, int y0, int R)
+// The scale walk of every direction for the P pixels of one lane, then the
+// aggregation and the stores.  `src` is the lane's LDS address: the float
+// tile value of its first pixel (kWalkExact, kWalkFma, kWalkRingFloat) or
+// the prefix value at its first pixel's column (kWalkRing, fixed point).
+template <int P, int MODE, int BODY, typename T>
+__device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, const DevKset &ks, const DevRings &rings,
+                                          const float *__restrict__ tab, float *__restrict__ y,
+                                          uint8_t *__restrict__ map, int W, int H, int pitch, int px0, int py)
 {
     constexpr bool EXACT = MODE == kWalkExact;
-    constexpr int LX = TW / P;
-    static_assert(LX * TH == kThreads, "tile shape must use 256 lanes");
-    const int lp = lds_pitch(TW, R, PM);
-    const int tid = threadIdx.x;
-
-    // Stage tile + halo; pixels outside the image read as zero, which is the
-    // reference's zero padding.
-    stage_tile<TW, TH>(tile, lp, x, W, H, pitch, x0, y0, R);
-    __syncthreads();
-
-    const int tx = tid % LX, ty = tid / LX;
-    const int px0 = x0 + tx * P, py = y0 + ty;
-    const float *lane = tile + (ty + R) * lp + tx * P + R;
     const int S = ks.S, K = ks.D * ks.S;
     const float *__restrict__ thr_t = tab + K;
     const float *__restrict__ inv_t = tab + 2 * K;
@@ -63,11 +50,12 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
     for (int d = 0; d < ks.D; ++d) {
         float lo[P], hi[P], best[P], binv[P];
         int sel[P];
-        float ring_sum[P];  // ring form: running sum of the tile over scales 0..s
-#pragma unroll
-        for (int p = 0; p < P; ++p) ring_sum[p] = 0.0f;
+        float ring_sum[P];    // float ring sums: running sum over scales 0..s
+        uint32_t ring_q[P];   // fixed-point ring sums
 #pragma unroll
         for (int p = 0; p < P; ++p) {
+            ring_sum[p] = 0.0f;
+            ring_q[p] = 0u;
             // Pixels outside the image start with an empty interval.
             lo[p] = inside[p] ? -INFINITY : INFINITY;
             hi[p] = inside[p] ? INFINITY : -INFINITY;
@@ -86,16 +74,23 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
             const int k = d * S + s;
             float acc[P];
             if constexpr (MODE == kWalkRing) {
-                // yhat_s = w_s * (sum over the support of scale s), built
-                // from the previous scale's sum plus this scale's ring.
-                if (alive) correlate_runs<P, false, 1, false>(lane, lp, rings.runs, nullptr, rings.kern[k], ring_sum);
+                // yhat_s = (w_s 2^-F) * (fixed-point sum over the support of
+                // scale s), the previous scale's sum plus this scale's ring;
+                // one rounding for the product, none hidden in the interval.
+                if (alive) prefix_runs<P>(src, lp, rings.runs, rings.kern[k], ring_q);
+                const float wq = rings.scale[k] * rings.qinv;
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
+            } else if constexpr (MODE == kWalkRingFloat) {
+                // yhat_s = w_s * (float sum over the support of scale s).
+                if (alive) correlate_runs<P, false, 1, false>(src, lp, rings.runs, nullptr, rings.kern[k], ring_sum);
                 const float w = rings.scale[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;
             } else {
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = 0.0f;
-                if (alive) correlate<P, EXACT, BODY>(lane, lp, ks, k, acc);
+                if (alive) correlate<P, EXACT, BODY>(src, lp, ks, k, acc);
             }
             const float thr = thr_t[k], inv = inv_t[k];
             bool any_ok = false;
@@ -131,6 +126,49 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
 #pragma unroll
     for (int p = 0; p < P; ++p)
         if (inside[p]) yrow[p] = num[p] / den[p];
+}
+
+// One output tile at (x0, y0) of an image.  `tile` is the work-group's LDS
+// buffer of lds_floats(TW + 1, TH, R, PM) words: the window (tile plus a halo
+// of R) with one leading column, which the fixed-point form's prefix table
+// needs (Q[r][0] = 0) and the float forms leave unused.  kWalkRing stages
+// the window as fixed-point prefix sums and walks it if every value lies
+// within the kernel set's range; otherwise, and for the other modes, the
+// window is staged as floats.  Every barrier is reached by every lane.
+template <int TW, int TH, int P, int MODE, int BODY, int PM>
+__device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks, const DevRings &rings,
+                                             const float *__restrict__ tab, const float *__restrict__ x,
+                                             float *__restrict__ y, uint8_t *__restrict__ map, int W, int H,
+                                             int pitch, int x0, int y0, int R)
+{
+    constexpr int LX = TW / P;
+    static_assert(LX * TH == kThreads, "tile shape must use 256 lanes");
+    const int lp = lds_pitch(TW + 1, R, PM);
+    const int tid = threadIdx.x;
+    const int tx = tid % LX, ty = tid / LX;
+    const int px0 = x0 + tx * P, py = y0 + ty;
+    // LDS offset of window column R + tx*P of row ty + R, not counting the
+    // leading column.  Past the leading column (the float tile) it is the
+    // lane's first pixel; in the prefix table it is Q[.][R + tx*P], the left
+    // boundary of a run that starts at that pixel.
+    const int lane_off = (ty + R) * lp + tx * P + R;
+    if constexpr (MODE == kWalkRing) {
+        uint32_t *q = (uint32_t *)tile;
+        if (stage_prefix<TW, TH>(q, lp, x, W, H, pitch, x0, y0, R, rings.qscale, rings.qrange)) {
+            walk_lane<P, kWalkRing, BODY>(q + lane_off, lp, ks, rings, tab, y, map, W, H, pitch, px0, py);
+            return;
+        }
+        // A value of the window lies outside the fixed-point range (or is
+        // not finite): float ring sums.  Every lane takes this branch.
+        stage_tile<TW, TH>(tile + 1, lp, x, W, H, pitch, x0, y0, R);
+        __syncthreads();
+        walk_lane<P, kWalkRingFloat, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, y, map, W, H, pitch, px0, py);
+    } else {
+        // Pixels outside the image read as zero: the reference's zero padding.
+        stage_tile<TW, TH>(tile + 1, lp, x, W, H, pitch, x0, y0, R);
+        __syncthreads();
+        walk_lane<P, MODE, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, y, map, W, H, pitch, px0, py);
+    }
 }
 
 // Plain form: one launch per image.  Work-groups are dealt to the 8 XCDs in
@@ -185,7 +223,7 @@ hipError_t plain_launch(const DevKset &ks, const DevRings &rings, const ImgDesc 
     const int tiles_y = (img.H + sh.th - 1) / sh.th;
     const int num_tiles = img.tiles_x * tiles_y;
     const int per_xcd = (num_tiles + 7) / 8;
-    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R, sh.pm);
+    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm);
     hipLaunchKernelGGL((fwd_plain_kernel<SH, MODE>), dim3(8 * per_xcd), dim3(kThreads), lds, stream, ks, rings, img,
                        R, num_tiles, per_xcd);
     return hipGetLastError();
@@ -198,6 +236,7 @@ hipError_t plain_dispatch(WalkMode mode, const DevKset &ks, const DevRings &ring
     switch (mode) {
     case kWalkExact: return plain_launch<SH, kWalkExact>(ks, rings, img, R, stream);
     case kWalkRing: return plain_launch<SH, kWalkRing>(ks, rings, img, R, stream);
+    case kWalkRingFloat: return plain_launch<SH, kWalkRingFloat>(ks, rings, img, R, stream);
     default: return plain_launch<SH, kWalkFma>(ks, rings, img, R, stream);
     }
 }
@@ -207,7 +246,7 @@ hipError_t batched_launch(const DevKset &ks, const DevRings &rings, const ImgDes
                           int num_tiles, int *counter, int R, int grid, hipStream_t stream)
 {
     constexpr TileShape sh = kShapes[SH];
-    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw, sh.th, R, sh.pm);
+    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm);
     hipLaunchKernelGGL((fwd_batched_kernel<SH, MODE>), dim3(grid), dim3(kThreads), lds, stream, ks, rings, imgs, tiles,
                        num_tiles, counter, R);
     return hipGetLastError();
@@ -221,6 +260,8 @@ hipError_t batched_dispatch(WalkMode mode, const DevKset &ks, const DevRings &ri
     switch (mode) {
     case kWalkExact: return batched_launch<SH, kWalkExact>(ks, rings, imgs, tiles, num_tiles, counter, R, grid, stream);
     case kWalkRing: return batched_launch<SH, kWalkRing>(ks, rings, imgs, tiles, num_tiles, counter, R, grid, stream);
+    case kWalkRingFloat:
+        return batched_launch<SH, kWalkRingFloat>(ks, rings, imgs, tiles, num_tiles, counter, R, grid, stream);
     default: return batched_launch<SH, kWalkFma>(ks, rings, imgs, tiles, num_tiles, counter, R, grid, stream);
     }
 }
@@ -228,9 +269,9 @@ hipError_t batched_dispatch(WalkMode mode, const DevKset &ks, const DevRings &ri
 template <int SH>
 int occupancy(int R)
 {
-    const size_t lds = sizeof(float) * (size_t)lds_floats(kShapes[SH].tw, kShapes[SH].th, R, kShapes[SH].pm);
+    const size_t lds = sizeof(float) * (size_t)lds_floats(kShapes[SH].tw + 1, kShapes[SH].th, R, kShapes[SH].pm);
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, kWalkFma>, kThreads, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, kWalkRing>, kThreads, lds) !=
         hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;

@@ -181,6 +181,213 @@ __device__ __forceinline__ void stage_tile(float *__restrict__ tile, int lp, con
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fixed-point ring sums.  For a kernel set in ring form (uniform weight w_s
+// per kernel, nested supports) the estimate of scale s is w_s times the sum
+// of the tile over the support.  Tile values are taken in fixed point,
+// q(v) = rint(v * 2^F), so sums are exact 32-bit integers, independent of
+// order, and a row run's sum is a difference of two row-prefix values:
+//   sum_{c=a}^{b} q(x[r][c]) = Q[r][b + 1] - Q[r][a],  Q[r][j] = sum_{c<j} q(x[r][c]).
+// F is chosen per kernel set so that any support sum of values below the
+// set's range bound fits in 31 bits; a window holding a value outside the
+// range (or not finite) is walked with float ring sums instead.  All integer
+// arithmetic is modulo 2^32; the final sums are exact.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t quantize(float v, float qscale)
+{
+    return (uint32_t)(int)__builtin_rintf(v * qscale);
+}
+
+// a * b rounded once (never fused into a following add).
+__device__ __forceinline__ float mul_rounded(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave: row_shr 1, 2, 4, 8
+// within each row of 16 lanes, then row_bcast15 / row_bcast31 carry the row
+// totals forward.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ uint32_t dpp_or_zero(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, true);
+}
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
+{
+    v += dpp_or_zero<0x111>(v);
+    v += dpp_or_zero<0x112>(v);
+    v += dpp_or_zero<0x114>(v);
+    v += dpp_or_zero<0x118>(v);
+    v += dpp_or_zero<0x142, 0xa>(v);
+    v += dpp_or_zero<0x143, 0xc>(v);
+    return v;
+}
+
+// Work-group AND of v, returned to every lane as a wave-uniform value; also
+// a barrier.  Built from wave ballots and a __syncthreads: a vote through
+// __syncthreads_and would act as a memory clobber for the compiler, and every
+// kernel-table load after it would become a vector load.  The vote slots are
+// rewritten by the next call only after the caller's next barrier.
+__device__ __forceinline__ bool block_all(bool v)
+{
+    __shared__ int votes[kThreads / 64];
+    const bool w = __all(v);
+    if ((threadIdx.x & 63) == 0) votes[threadIdx.x >> 6] = w;
+    __syncthreads();
+    int r = 1;
+#pragma unroll
+    for (int i = 0; i < kThreads / 64; ++i) r &= votes[i];
+    return __builtin_amdgcn_readfirstlane(r) != 0;
+}
+
+// Stage the (TW + 2R) x (TH + 2R) window (as stage_tile) as row-prefix sums
+// of fixed-point values: q[r * lp + j] = Q[r][j] for j = 0 .. TW + 2R
+// (Q[r][0] = 0).  Each wave takes whole rows, two values per lane, and scans
+// them with wave_inclusive_sum.  Returns, to every thread of the work-group,
+// whether every value of the window lies strictly within +-qrange (this is
+// also the barrier after staging).
+template <int TW, int TH>
+__device__ __forceinline__ bool stage_prefix(uint32_t *__restrict__ q, int lp, const float *__restrict__ x, int W,
+                                             int H, int pitch, int x0, int y0, int R, float qscale, float qrange)
+{
+    static_assert(TW + 2 * 31 <= 128, "a window row must fit two values per lane");
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int fw = TW + 2 * R, fh = TH + 2 * R;
+    const auto gx = as_global(x);
+    const int c = 2 * lane, gc = x0 - R + c;
+    const bool in0 = c < fw && gc >= 0 && gc < W, in1 = c + 1 < fw && gc + 1 >= 0 && gc + 1 < W;
+    bool fits = true;
+    for (int r = tid >> 6; r < fh; r += kThreads / 64) {
+        const int gy = y0 - R + r;
+        const bool row_in = gy >= 0 && gy < H;
+        const auto xrow = gx + (size_t)(row_in ? gy : 0) * pitch;
+        const float v0 = row_in && in0 ? xrow[gc] : 0.0f;
+        const float v1 = row_in && in1 ? xrow[gc + 1] : 0.0f;
+        fits = fits && __builtin_fabsf(v0) < qrange && __builtin_fabsf(v1) < qrange;
+        const uint32_t q1 = quantize(v1, qscale);
+        const uint32_t incl = wave_inclusive_sum(quantize(v0, qscale) + q1);
+        uint32_t *__restrict__ row = q + r * lp;
+        if (c < fw) row[1 + c] = incl - q1;
+        if (c + 1 < fw) row[2 + c] = incl;
+        if (lane == 0) row[0] = 0u;
+    }
+    return block_all(fits);
+}
+
+// t[p] = sum of a run of length L for the P pixels whose left-boundary
+// prefix values start at src: t[p] = src[L + p] - src[p].
+template <int L, int P>
+__device__ __forceinline__ void prefix_run(const uint32_t *__restrict__ src, uint32_t (&t)[P])
+{
+    uint32_t v[L + P];
+#pragma unroll
+    for (int i = 0; i < L + P; ++i) v[i] = src[i];
+#pragma unroll
+    for (int p = 0; p < P; ++p) t[p] = v[L + p] - v[p];
+}
+template <int P>
+__device__ __forceinline__ void prefix_run_long(const uint32_t *__restrict__ src, int len, uint32_t (&t)[P])
+{
+    uint32_t a[P], b[P];
+    const uint32_t *__restrict__ e = src + len;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        a[p] = src[p];
+        b[p] = e[p];
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) t[p] = b[p] - a[p];
+}
+
+// rs[p] += fixed-point sum over the runs kr of the tile, for the P pixels
+// whose left-boundary prefix values start at `lane` (prefix table as written
+// by stage_prefix).  A run of length L <= P reads L + P prefix values, a
+// longer one 2P; either way two integer adds per pixel.
+template <int P>
+__device__ __forceinline__ void prefix_runs(const uint32_t *__restrict__ lane, int lp, const int4 *__restrict__ runs,
+                                            int2 kr, uint32_t (&rs)[P])
+{
+    for (int r = 0; r < kr.y; ++r) {
+        const int4 run = runs[kr.x + r];
+        const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
+        const uint32_t *__restrict__ src = lane + off;
+        uint32_t t[P];
+        switch (run.z) {
+#define LPA_PREFIX_CASE(L)                                                                                         \
+    case L: prefix_run<L, P>(src, t); break;
+            LPA_PREFIX_CASE(1) LPA_PREFIX_CASE(2) LPA_PREFIX_CASE(3) LPA_PREFIX_CASE(4)
+            LPA_PREFIX_CASE(5) LPA_PREFIX_CASE(6) LPA_PREFIX_CASE(7) LPA_PREFIX_CASE(8)
+#undef LPA_PREFIX_CASE
+        default: prefix_run_long<P>(src, run.z, t); break;
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) rs[p] += t[p];
+    }
+}
+
+// The same fixed-point sums from a float tile (quantizing each value read):
+// rq[p] += sum over the runs kr of q(tile[row, col + p + j]).
+template <int L, int P>
+__device__ __forceinline__ void run_taps_q(const float *__restrict__ src, float qscale, uint32_t (&rq)[P])
+{
+    uint32_t v[L + P - 1];
+#pragma unroll
+    for (int i = 0; i < L + P - 1; ++i) v[i] = quantize(src[i], qscale);
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+#pragma unroll
+        for (int p = 0; p < P; ++p) rq[p] += v[j + p];
+}
+template <int P>
+__device__ __forceinline__ void quantized_runs(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
+                                               int2 kr, float qscale, uint32_t (&rq)[P])
+{
+    for (int r = 0; r < kr.y; ++r) {
+        const int4 run = runs[kr.x + r];
+        const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
+        const float *__restrict__ src = lane + off;
+        switch (run.z) {
+#define LPA_QRUN_CASE(L)                                                                                           \
+    case L: run_taps_q<L, P>(src, qscale, rq); break;
+            LPA_QRUN_CASE(1) LPA_QRUN_CASE(2) LPA_QRUN_CASE(3) LPA_QRUN_CASE(4) LPA_QRUN_CASE(5) LPA_QRUN_CASE(6)
+            LPA_QRUN_CASE(7) LPA_QRUN_CASE(8) LPA_QRUN_CASE(9) LPA_QRUN_CASE(10) LPA_QRUN_CASE(11)
+            LPA_QRUN_CASE(12) LPA_QRUN_CASE(13) LPA_QRUN_CASE(14) LPA_QRUN_CASE(15) LPA_QRUN_CASE(16)
+            LPA_QRUN_CASE(17) LPA_QRUN_CASE(18) LPA_QRUN_CASE(19) LPA_QRUN_CASE(20) LPA_QRUN_CASE(21)
+            LPA_QRUN_CASE(22) LPA_QRUN_CASE(23) LPA_QRUN_CASE(24) LPA_QRUN_CASE(25) LPA_QRUN_CASE(26)
+            LPA_QRUN_CASE(27) LPA_QRUN_CASE(28) LPA_QRUN_CASE(29) LPA_QRUN_CASE(30) LPA_QRUN_CASE(31)
+            LPA_QRUN_CASE(32)
+#undef LPA_QRUN_CASE
+        default: break;
+        }
+    }
+}
+
+// stage_tile, also returning to every thread whether every staged value lies
+// strictly within +-qrange (this is also the barrier after staging).
+template <int TW, int TH>
+__device__ __forceinline__ bool stage_tile_fits(float *__restrict__ tile, int lp, const float *__restrict__ x, int W,
+                                                int H, int pitch, int x0, int y0, int R, float qrange)
+{
+    const int tid = threadIdx.x;
+    const int fw = TW + 2 * R, fh = TH + 2 * R;
+    const auto gx = as_global(x);
+    bool fits = true;
+    for (int r = tid >> 6; r < fh; r += kThreads / 64) {
+        const int gy = y0 - R + r;
+        const bool row_in = gy >= 0 && gy < H;
+        const auto xrow = gx + (size_t)(row_in ? gy : 0) * pitch;
+        for (int c = tid & 63; c < fw; c += 64) {
+            const int gc = x0 - R + c;
+            const float v = (row_in && gc >= 0 && gc < W) ? xrow[gc] : 0.0f;
+            fits = fits && __builtin_fabsf(v) < qrange;
+            tile[r * lp + c] = v;
+        }
+    }
+    return block_all(fits);
+}
+
 }  // namespace lpa
 
 #endif

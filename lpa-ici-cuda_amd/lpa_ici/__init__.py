@@ -10,12 +10,12 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (DEFAULT_KERNELS, FWD_EXACT, FWD_GRAPH, FWD_PLAIN, FWD_WEIGHTED, GAUSSIAN_M20_KERNELS, KERNEL_DIR, LIB_PATH,
-                   LpaError, LpaImage, check, lib)
+from ._lib import (DEFAULT_KERNELS, FWD_EXACT, FWD_GRAPH, FWD_PLAIN, FWD_RING_FLOAT, FWD_WEIGHTED, GAUSSIAN_M20_KERNELS,
+                   KERNEL_DIR, LIB_PATH, LpaError, LpaImage, check, lib)
 
 __all__ = [
     "DEFAULT_KERNELS", "GAUSSIAN_M20_KERNELS", "KERNEL_DIR", "LIB_PATH", "FWD_PLAIN", "FWD_EXACT", "FWD_GRAPH",
-    "FWD_WEIGHTED",
+    "FWD_WEIGHTED", "FWD_RING_FLOAT",
     "LpaError", "KernelSet", "Plan", "load_kernels", "kernel_variances", "make_noisy", "psnr", "synth_image",
     "load_image", "write_hdr", "denoise", "backward", "image_struct", "tile_configs", "lib",
 ]
@@ -95,6 +95,28 @@ class KernelSet:
                 k = self.kernel(d, s)
                 pos = {(r - k.shape[0] // 2, c - k.shape[1] // 2) for r, c in zip(*np.nonzero(k))}
                 out[d * self.S + s] = len(pos - prev)
+                prev = pos
+        return out
+
+    def ring_runs_per_kernel(self):
+        """Row runs of each kernel's ring (ring form only), cut at 32 taps as
+        the packer cuts them: the fixed-point walk does two integer adds per
+        pixel per run."""
+        out = np.zeros(self.K, np.int64)
+        for d in range(self.D):
+            prev = set()
+            for s in range(self.S):
+                k = self.kernel(d, s)
+                pos = {(r - k.shape[0] // 2, c - k.shape[1] // 2) for r, c in zip(*np.nonzero(k))}
+                ring = sorted(pos - prev)
+                n, i = 0, 0
+                while i < len(ring):
+                    j = i + 1
+                    while j < len(ring) and ring[j][0] == ring[i][0] and ring[j][1] == ring[j - 1][1] + 1 and j - i < 32:
+                        j += 1
+                    n += 1
+                    i = j
+                out[d * self.S + s] = n
                 prev = pos
         return out
 

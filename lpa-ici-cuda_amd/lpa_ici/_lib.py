@@ -21,7 +21,7 @@ GAUSSIAN_M20_KERNELS = os.path.join(KERNEL_DIR, "lpa_kernels_m_2_0_d_16_h12_1_2_
 
 LPA_OK, LPA_EINVAL, LPA_ETOOBIG, LPA_ENOMEM, LPA_EHIP, LPA_EIO = range(6)
 STATUS_NAMES = {0: "LPA_OK", 1: "LPA_EINVAL", 2: "LPA_ETOOBIG", 3: "LPA_ENOMEM", 4: "LPA_EHIP", 5: "LPA_EIO"}
-FWD_PLAIN, FWD_EXACT, FWD_GRAPH, FWD_WEIGHTED = 1, 2, 4, 8
+FWD_PLAIN, FWD_EXACT, FWD_GRAPH, FWD_WEIGHTED, FWD_RING_FLOAT = 1, 2, 4, 8, 16
 
 
 class LpaError(RuntimeError):
@@ -49,7 +49,7 @@ class LpaImage(ctypes.Structure):
 class KsetInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in
                 ("num_dirs", "num_scales", "num_runs", "num_weights", "num_nonzero", "max_radius", "ring_form",
-                 "num_ring_taps")]
+                 "num_ring_taps", "ring_frac_bits")] + [("ring_range", ctypes.c_float)]
 
 
 class TileConfig(ctypes.Structure):

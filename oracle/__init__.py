@@ -38,10 +38,12 @@ def _lib():
         lib.oracle_v1_0.argtypes = common + [_P, _F]
         lib.oracle_walk_f32.argtypes = common + [_P, _F, _P, _P, _P, _I]
         lib.oracle_walk_f64.argtypes = common + [_D, _D, _P, _P, _P, _I]
+        lib.oracle_walk_q.argtypes = common + [_P, _F, _I, _F, _P, _P, _I]
         lib.oracle_adjoint_f64.argtypes = [_P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _D, _P, _P, _P]
         lib.oracle_useful_taps.argtypes = [_P, ctypes.c_size_t, _I, _I, _P]
         lib.oracle_useful_taps.restype = ctypes.c_longlong
-        for f in ("oracle_v0_0", "oracle_v1_0", "oracle_walk_f32", "oracle_walk_f64", "oracle_adjoint_f64"):
+        for f in ("oracle_v0_0", "oracle_v1_0", "oracle_walk_f32", "oracle_walk_q", "oracle_walk_f64",
+                  "oracle_adjoint_f64"):
             getattr(lib, f).restype = _I
         _oracle = lib
     return _oracle
@@ -91,6 +93,25 @@ def walk_f32(x, ks, variances, gamma, threads=None, want_est=False):
                               _ptr(marg), _ptr(est), int(threads)):
         raise MemoryError
     return (y, maps, marg, est) if want_est else (y, maps, marg)
+
+
+def walk_q(x, ks, variances, gamma, frac_bits, qrange, threads=None, want_est=False):
+    """Fixed-point ring-sum walk (uniform-weight kernels).  Returns (y, scale_maps[, est])."""
+    x, W, H = _img(x)
+    y = np.zeros((H, W), np.float32)
+    var = np.ascontiguousarray(variances, np.float32)
+    maps = np.zeros((ks.D, H, W), np.uint8)
+    est = np.zeros((ks.D, H, W), np.float32) if want_est else None
+    threads = threads or min(16, os.cpu_count() or 1)
+    rc = _lib().oracle_walk_q(_ptr(x), _ptr(y), W, H, ks.D, ks.S, *_kargs(ks), _ptr(var), float(gamma), int(frac_bits),
+                              float(qrange), _ptr(maps), _ptr(est), int(threads))
+    if rc == -2:
+        raise ValueError("a value lies outside the fixed-point range")
+    if rc == -3:
+        raise ValueError("kernel weights are not uniform")
+    if rc:
+        raise MemoryError
+    return (y, maps, est) if want_est else (y, maps)
 
 
 def walk_f64(x, ks, sigma, gamma, frozen=None, threads=None, want_est=False):

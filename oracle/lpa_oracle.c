@@ -27,6 +27,13 @@
  *                      FP64 error reference).  Pinned by closed forms
  *                      (constant image, impulse, single direction) and by
  *                      agreement with the f32 walk to ~1e-6.
+ *   oracle_walk_q      the fixed-point ring-sum walk, the HIP path's default
+ *                      fast walk for kernel sets with one uniform weight per
+ *                      kernel and nested supports (no reference counterpart:
+ *                      it computes the reference's estimator w * sum(x) with
+ *                      the sum in exact integers).  Pinned by the closed form
+ *                      of a constant image, an impulse (orientation), the
+ *                      f32 walk's tolerance bar and the FP64 walk.
  *   oracle_adjoint_f64 dL/dx and dL/dk of the frozen-selection forward.
  *                      Pinned by central finite differences of
  *                      oracle_walk_f64 with frozen scales (the output is
@@ -295,6 +302,78 @@ int oracle_walk_f32(const float* noisy, float* out, int W, int H, int D, int S,
         }
     }
     free(thr); free(inv);
+    free_taps(taps, K);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Fixed-point ring-sum walk                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* With q(v) = rintf(v * 2^F) and w_k the single nonzero weight of kernel k:
+ *   yhat_k(p) = fl( fl((float) sum_{t in support k} q(x[p+t])) * (w_k * 2^-F) ),
+ * the sum exact in integers (taps outside the image add q(0) = 0; the order
+ * of the taps does not matter), then the interval update with lo and hi
+ * rounded separately, the early exit and the aggregation of oracle_walk_f32.
+ * Returns -2 if some |x| >= range or x is not finite (the fixed point does
+ * not apply), -3 if a kernel's nonzero weights are not all equal, -1 on
+ * allocation failure.  Optional outputs [D][H][W]: scale_map, est. */
+int oracle_walk_q(const float* noisy, float* out, int W, int H, int D, int S,
+                  const float* weights, const int* offsets, const int* rows,
+                  const int* cols, const float* variances, float gamma,
+                  int frac_bits, float range, uint8_t* scale_map, float* est, int threads)
+{
+    int K = D * S;
+    size_t N = (size_t)W * H;
+    for (size_t i = 0; i < N; i++)
+        if (!(fabsf(noisy[i]) < range)) return -2;
+    tap_list* taps = make_all_taps(K, weights, offsets, rows, cols);
+    if (!taps) return -1;
+    float* thr = (float*)malloc(sizeof(float) * K);
+    float* inv = (float*)malloc(sizeof(float) * K);
+    float* wq = (float*)malloc(sizeof(float) * K);
+    int32_t* q = (int32_t*)malloc(sizeof(int32_t) * N);
+    if (!thr || !inv || !wq || !q) return -1;
+    const float qscale = ldexpf(1.0f, frac_bits), qinv = ldexpf(1.0f, -frac_bits);
+    for (int k = 0; k < K; k++) {
+        thr[k] = gamma * sqrtf(variances[k]);
+        inv[k] = 1.0f / variances[k];
+        for (int j = 0; j < taps[k].n; j++)
+            if (memcmp(&taps[k].w[j], &taps[k].w[0], sizeof(float)) != 0) { free_taps(taps, K); return -3; }
+        wq[k] = (taps[k].n ? taps[k].w[0] : 0.0f) * qinv;
+    }
+    for (size_t i = 0; i < N; i++) q[i] = (int32_t)rintf(noisy[i] * qscale);
+    (void)threads;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (int r = 0; r < H; r++) {
+        for (int c = 0; c < W; c++) {
+            size_t i = (size_t)r * W + c;
+            float num = 0.0f, den = 0.0f;
+            for (int d = 0; d < D; d++) {
+                float L = -INFINITY, U = INFINITY, best = 0.0f, bi = 0.0f;
+                int sel = 0;
+                for (int s = 0; s < S; s++) {
+                    int k = d * S + s;
+                    int64_t sum = 0;
+                    for (int j = 0; j < taps[k].n; j++) {
+                        int ir = r + taps[k].dr[j], ic = c + taps[k].dc[j];
+                        if (ir >= 0 && ir < H && ic >= 0 && ic < W) sum += q[(size_t)ir * W + ic];
+                    }
+                    float v = (float)(int32_t)sum * wq[k];
+                    L = fmaxf_ref(L, v - thr[k]);
+                    U = fminf_ref(U, v + thr[k]);
+                    if (U < L) break;
+                    best = v; bi = inv[k]; sel = s;
+                }
+                num += best * bi;
+                den += bi;
+                if (scale_map) scale_map[(size_t)d * N + i] = (uint8_t)sel;
+                if (est) est[(size_t)d * N + i] = best;
+            }
+            out[i] = num / den;
+        }
+    }
+    free(thr); free(inv); free(wq); free(q);
     free_taps(taps, K);
     return 0;
 }

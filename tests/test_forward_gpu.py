@@ -4,10 +4,13 @@ reference's v0_0 path) and with fixtures made by the reference itself.
 Bars (the reference's algorithm in FP32 on the same seeded noisy image):
   * exact mode (separately rounded mul/add, the reference's tap order):
     bit-identical output and scale maps;
-  * fast mode (FMA): per-direction scale agreement >= 99.9 %, every
+  * fast modes (fixed-point ring sums, the default for ring-form sets; float
+    ring sums; weighted FMA): per-direction scale agreement >= 99.9 %, every
     disagreement at an interval margin below 1e-6, |dy| <= 1e-5 wherever all
     directions agree, |dPSNR| <= 0.005 dB; the error against the FP64 oracle
-    is printed for every case.
+    is printed for every case;
+  * the fixed-point walk is also bit-identical to the oracle's restatement of
+    it (integer sums do not depend on the order of the taps).
 """
 import numpy as np
 import pytest
@@ -78,7 +81,16 @@ def test_exact_cameraman_bitexact(ks):
     assert np.array_equal(maps, mo)
 
 
-@pytest.mark.parametrize("flags,label", [(0, "ring sums"), (lpa_ici.FWD_WEIGHTED, "weighted FMA")])
+WALKS = [(0, "fixed-point ring sums"), (lpa_ici.FWD_RING_FLOAT, "float ring sums"),
+         (lpa_ici.FWD_WEIGHTED, "weighted FMA")]
+
+
+def fixed_point(ks):
+    info = ks.info()
+    return info["ring_frac_bits"], info["ring_range"]
+
+
+@pytest.mark.parametrize("flags,label", WALKS)
 def test_fast_parity_config1_512(ks, flags, label):
     """Config 1: one 512x512 synthetic image, sigma 20/255, gamma 2; the
     default set has the ring form, so both fast walks are held to the bar."""
@@ -97,7 +109,7 @@ def test_fast_parity_gaussian_m20(ks):
     check_fast_parity(ksg, x, clean, y, maps, "256x192 m20 gaussian")
 
 
-@pytest.mark.parametrize("flags,label", [(0, "ring sums"), (lpa_ici.FWD_WEIGHTED, "weighted FMA")])
+@pytest.mark.parametrize("flags,label", WALKS)
 def test_fast_parity_barbara(ks, flags, label):
     """The reference's published case: PSNR 25.0736 on barbara."""
     clean = lpa_ici.load_image(lpa_ici.KERNEL_DIR + "/../../tests/golden/images/barbara.png")
@@ -105,6 +117,46 @@ def test_fast_parity_barbara(ks, flags, label):
     y, maps = run(ks, x, flags=flags)
     check_fast_parity(ks, x, clean, y, maps, f"barbara, {label}")
     assert round(lpa_ici.psnr(clean, y), 4) == 25.0736
+
+
+@pytest.mark.parametrize("w,h", [(512, 512), (1, 1), (9, 1), (33, 65), (129, 47), (300, 200)])
+def test_fixed_point_bitexact(ks, w, h):
+    """The default walk of a ring-form set equals the oracle's fixed-point
+    restatement bit for bit: output and scale maps."""
+    F, rng = fixed_point(ks)
+    clean = lpa_ici.synth_image(w, h, 1)
+    x = lpa_ici.make_noisy(clean, SIGMA, SEED)
+    y, maps = run(ks, x)
+    yo, mo = oracle.walk_q(x, ks, ks.variances(SIGMA), GAMMA, F, rng)
+    assert np.array_equal(maps, mo)
+    assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
+
+
+def test_fixed_point_fallback_windows(ks):
+    """A block of values outside the fixed-point range: windows that reach it
+    take float ring sums, the rest stay in fixed point.  Far from the block
+    the output equals the fixed-point restatement bit for bit; everywhere it
+    meets the fast-mode bar against the reference-order FP32 walk."""
+    F, rng = fixed_point(ks)
+    x = noisy_synth(512, 384, seed=17)
+    x[40:50, 40:50] = 3.0 * rng
+    y, maps = run(ks, x)
+    check_fast_parity(ks, x, None, y, maps, "512x384 with an out-of-range block")
+    yo, mo = oracle.walk_q(x, ks, ks.variances(SIGMA), GAMMA, F, np.inf)
+    # a window (tile plus halo) that holds the block is at most the largest
+    # tile side plus the halo away from it
+    reach = max(max(c["tile_w"], c["tile_h"]) for c in lpa_ici.tile_configs()) + 2 * 31
+    far = np.ones(x.shape, bool)
+    far[:50 + reach, :50 + reach] = False
+    assert far.sum() > 100000
+    assert np.array_equal(y.view(np.uint32)[far], yo.view(np.uint32)[far])
+    assert np.array_equal(maps[:, far], mo[:, far])
+    # every window out of range: float ring sums everywhere
+    xs = x[:128, :128] + np.float32(3.0)
+    y2, m2 = run(ks, xs)
+    check_fast_parity(ks, xs, None, y2, m2, "offset +3 (float ring sums)")
+    y3, m3 = run(ks, xs, flags=lpa_ici.FWD_RING_FLOAT)
+    assert np.array_equal(y2, y3) and np.array_equal(m2, m3)
 
 
 @pytest.mark.parametrize("shape", range(len(lpa_ici.tile_configs())))

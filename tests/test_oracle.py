@@ -262,3 +262,112 @@ def test_useful_taps_count(ks):
     assert oracle.useful_taps(maps, nnz) == brute
     from lpa_ici.metrics import useful_taps
     assert useful_taps(maps, nnz, ks.S) == brute
+
+
+# ---------------------------------------------------------------------------
+# the fixed-point ring-sum walk (the HIP path's default fast walk)
+# ---------------------------------------------------------------------------
+
+def fixed_point(ks):
+    info = ks.info()
+    assert info["ring_form"] == 1
+    return info["ring_frac_bits"], info["ring_range"]
+
+
+def test_fixed_point_parameters(ks):
+    """F is the largest power with n_max * 2 * 2^F < 2^31 (n_max = 204 taps for
+    the default set), and the range keeps every support sum inside 31 bits."""
+    F, rng = fixed_point(ks)
+    nmax = max(int((ks.weights[ks.offsets[k]:ks.offsets[k] + ks.rows[k] * ks.cols[k]] != 0).sum())
+               for k in range(ks.K))
+    assert nmax == 204 and F == 22
+    assert nmax * 2 * 2.0 ** F < 2 ** 31 <= nmax * 2 * 2.0 ** (F + 1)
+    assert nmax * (rng * 2.0 ** F + 0.5) < 2 ** 31 and 2.5 < rng < 2.6
+
+
+def test_walk_q_constant_image_closed_form(ks):
+    """A constant c away from the borders: every direction reaches the top
+    scale and yhat = fl(fl(n * q(c)) * (w * 2^-F)) with n the support size.
+    The same check fails when one tap of a support is dropped."""
+    F, rng = fixed_point(ks)
+    c = np.float32(0.3)
+    x = np.full((80, 80), c, np.float32)
+    qc = int(np.rint(np.float64(c) * 2.0 ** F))
+    inner = slice(32, 48)
+
+    def expected(kset, k):
+        m = kset.weights[kset.offsets[k]:kset.offsets[k] + kset.rows[k] * kset.cols[k]]
+        n = int((m != 0).sum())
+        w = m[m != 0][0]
+        return np.float32(np.int64(n * qc)) * (np.float32(w) * np.float32(2.0 ** -F))
+
+    _, maps, est = oracle.walk_q(x, ks, ks.variances(SIGMA), GAMMA, F, rng, want_est=True)
+    assert np.all(maps[:, inner, inner] == ks.S - 1)
+    for d in range(ks.D):
+        e = expected(ks, d * ks.S + ks.S - 1)
+        assert np.all(bits(est[d, inner, inner]) == bits(e)), d
+
+    class Dropped:  # kernel (0, S-1) loses one tap
+        D, S = ks.D, ks.S
+        offsets, rows, cols = ks.offsets, ks.rows, ks.cols
+        weights = ks.weights.copy()
+    k = ks.S - 1
+    nz = np.flatnonzero(Dropped.weights[ks.offsets[k]:ks.offsets[k] + ks.rows[k] * ks.cols[k]])
+    Dropped.weights[ks.offsets[k] + nz[-1]] = 0.0
+    _, _, est2 = oracle.walk_q(x, Dropped, ks.variances(SIGMA), GAMMA, F, rng, want_est=True)
+    assert not np.all(bits(est2[0, inner, inner]) == bits(expected(ks, k)))
+
+
+def test_walk_q_impulse_orientation():
+    """Uniform 2-tap wedges (right of the centre; below the centre): an impulse
+    at (4, 4) shows up at the pixels the reflected supports reach."""
+    class Uniform:
+        D, S = 2, 2
+        mats = [np.array([[1.0]], np.float32), np.zeros((3, 3), np.float32),
+                np.array([[1.0]], np.float32), np.zeros((3, 3), np.float32)]
+        mats[1][1, 1:] = 0.5
+        mats[3][1:, 1] = 0.5
+        rows = np.array([m.shape[0] for m in mats], np.int32)
+        cols = np.array([m.shape[1] for m in mats], np.int32)
+        offsets = np.cumsum([0] + [m.size for m in mats[:-1]]).astype(np.int32)
+        weights = np.concatenate([m.ravel() for m in mats]).astype(np.float32)
+    F, rng = 22, 2.5
+    x = np.zeros((9, 9), np.float32)
+    x[4, 4] = 0.7
+    var = np.full(4, 100.0, np.float32)  # wide intervals: scale 1 is always taken
+    _, maps, est = oracle.walk_q(x, Uniform, var, 1.0, F, rng, want_est=True)
+    assert np.all(maps == 1)
+    hit = np.float32(np.rint(0.7 * 2.0 ** F)) * np.float32(0.5 * 2.0 ** -F)
+    e0 = np.zeros((9, 9), np.float32); e0[4, 3] = e0[4, 4] = hit   # pixel p sees x[p], x[p + (0, 1)]
+    e1 = np.zeros((9, 9), np.float32); e1[3, 4] = e1[4, 4] = hit   # pixel p sees x[p], x[p + (1, 0)]
+    assert np.array_equal(bits(est[0]), bits(e0)) and np.array_equal(bits(est[1]), bits(e1))
+
+
+def test_walk_q_meets_the_bar_against_f32(ks):
+    """Against the reference-order FP32 walk: the fast-mode bar (per-direction
+    scale agreement >= 99.9 %, disagreements only at margins < 1e-6,
+    |dy| <= 1e-5 where every direction agrees), and close to FP64."""
+    F, rng = fixed_point(ks)
+    x = noisy_synth(160, 120, seed=12)
+    var = ks.variances(SIGMA)
+    yq, mq = oracle.walk_q(x, ks, var, GAMMA, F, rng)
+    yf, mf, marg = oracle.walk_f32(x, ks, var, GAMMA)
+    agree = mq == mf
+    assert agree.reshape(ks.D, -1).mean(axis=1).min() >= 0.999
+    assert np.all(marg[~agree] < 1e-6)
+    all_agree = agree.all(axis=0)
+    assert np.abs(yq.astype(np.float64) - yf)[all_agree].max() <= 1e-5
+    y64, m64 = oracle.walk_f64(x, ks, SIGMA, GAMMA)
+    same = (mq == m64).all(axis=0)
+    assert np.abs(yq - y64)[same].max() < 2e-6
+
+
+def test_walk_q_refuses_out_of_range(ks):
+    F, rng = fixed_point(ks)
+    x = noisy_synth(20, 20, seed=1)
+    x[3, 3] = rng
+    with pytest.raises(ValueError):
+        oracle.walk_q(x, ks, ks.variances(SIGMA), GAMMA, F, rng)
+    x[3, 3] = np.nan
+    with pytest.raises(ValueError):
+        oracle.walk_q(x, ks, ks.variances(SIGMA), GAMMA, F, rng)
