@@ -147,7 +147,7 @@ __device__ __forceinline__ void grad_piece(const float *__restrict__ src, const 
     if constexpr (YHAT) {
 #pragma unroll
         for (int j = 0; j < L; ++j) {
-            const float c = w[j];
+            const float c = as_const(w)[j];
 #pragma unroll
             for (int p = 0; p < GP; ++p) yacc[p] = __builtin_fmaf(c, v[j + p], yacc[p]);
         }
@@ -199,11 +199,11 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
         __syncthreads();
         const int t = next;
         if (t >= num_tiles) break;
-        const uint2 te = tiles[t];
-        const BwdDesc bd = imgs[te.x];
+        const uint2 te = load_const(tiles + t);
+        const BwdDesc bd = load_const_struct(imgs + te.x);
         const ImgDesc &img = bd.img;
         const int x0 = (int)(te.y & 0xffffu) * GW, y0 = (int)(te.y >> 16) * GH;
-        for (int i = tid; i < 3 * K; i += kThreads) tab[i] = img.tab[i];
+        for (int i = tid; i < 3 * K; i += kThreads) tab[i] = as_const(img.tab)[i];
         // fixed: the forward's fixed-point ring sums apply to this window
         // (every value within the range); the test is also the barrier.
         bool fixed = false;
@@ -263,15 +263,15 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
                     if (!__any(need)) break;
                     const int k = d * S + s;
                     if (fixed) {
-                        if (need) quantized_runs<GP>(lane_x, lp, rings.runs, rings.kern[k], rings.qscale, rq);
-                        const float wq = rings.scale[k] * rings.qinv;
+                        if (need) quantized_runs<GP>(lane_x, lp, rings.runs, load_const(rings.kern + k), rings.qscale, rq);
+                        const float wq = as_const(rings.scale)[k] * rings.qinv;
 #pragma unroll
                         for (int p = 0; p < GP; ++p)
                             if (sd[p] == s) yh[p] = mul_rounded((float)(int)rq[p], wq);
                     } else {
                         if (need)
-                            correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr, rings.kern[k], rs);
-                        const float w = rings.scale[k];
+                            correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr, load_const(rings.kern + k), rs);
+                        const float w = as_const(rings.scale)[k];
 #pragma unroll
                         for (int p = 0; p < GP; ++p)
                             if (sd[p] == s) yh[p] = rs[p] * w;
@@ -291,9 +291,9 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
                     m[p] = sd[p] == s ? q[p] * inv : 0.0f;
                     yacc[p] = 0.0f;
                 }
-                const int2 kr = ks.kern[k];
+                const int2 kr = load_const(ks.kern + k);
                 for (int r = 0; r < kr.y; ++r) {
-                    const int4 run = ks.runs[kr.x + r];
+                    const int4 run = load_const(ks.runs + kr.x + r);
                     for (int j0 = 0; j0 < run.z; j0 += kGradPiece) {
                         const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y + j0);
                         const float *src = lane_x + off;
@@ -352,8 +352,8 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
         __syncthreads();
         const int t = next;
         if (t >= num_tiles) break;
-        const uint2 te = tiles[t];
-        const BwdDesc bd = imgs[te.x];
+        const uint2 te = load_const(tiles + t);
+        const BwdDesc bd = load_const_struct(imgs + te.x);
         const ImgDesc &img = bd.img;
         const int x0 = (int)(te.y & 0xffffu) * DXW, y0 = (int)(te.y >> 16) * DXH;
         const float *__restrict__ inv_t = img.tab + 2 * K;  // wave-uniform scalar loads
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
             for (int s = 0; s < S; ++s) {
                 int any = 0;
                 __syncthreads();  // previous correlation done with the map
-                const float inv = inv_t[d * S + s];
+                const float inv = as_const(inv_t)[d * S + s];
 #pragma unroll
                 for (int i = 0; i < kSlotRows; ++i)
 #pragma unroll

@@ -77,14 +77,14 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 // yhat_s = (w_s 2^-F) * (fixed-point sum over the support of
                 // scale s), the previous scale's sum plus this scale's ring;
                 // one rounding for the product, none hidden in the interval.
-                if (alive) prefix_runs<P>(src, lp, rings.runs, rings.kern[k], ring_q);
-                const float wq = rings.scale[k] * rings.qinv;
+                if (alive) prefix_runs<P>(src, lp, rings.runs, load_const(rings.kern + k), ring_q);
+                const float wq = as_const(rings.scale)[k] * rings.qinv;
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {
                 // yhat_s = w_s * (float sum over the support of scale s).
-                if (alive) correlate_runs<P, false, 1, false>(src, lp, rings.runs, nullptr, rings.kern[k], ring_sum);
-                const float w = rings.scale[k];
+                if (alive) correlate_runs<P, false, 1, false>(src, lp, rings.runs, nullptr, load_const(rings.kern + k), ring_sum);
+                const float w = as_const(rings.scale)[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;
             } else {
@@ -92,7 +92,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 for (int p = 0; p < P; ++p) acc[p] = 0.0f;
                 if (alive) correlate<P, EXACT, BODY>(src, lp, ks, k, acc);
             }
-            const float thr = thr_t[k], inv = inv_t[k];
+            const float thr = as_const(thr_t)[k], inv = as_const(inv_t)[k];
             bool any_ok = false;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
@@ -205,8 +205,8 @@ __global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, DevRi
         __syncthreads();
         const int t = next;
         if (t >= num_tiles) break;
-        const uint2 e = tiles[t];
-        const ImgDesc img = imgs[e.x];
+        const uint2 e = load_const(tiles + t);
+        const ImgDesc img = load_const_struct(imgs + e.x);
         process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W,
                                                         img.H, img.pitch, (int)(e.y & 0xffffu) * sh.tw,
                                                         (int)(e.y >> 16) * sh.th, R);

@@ -18,6 +18,45 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *as_global(T *p)
     return (__attribute__((address_space(1))) T *)p;
 }
 
+// Read-only tables (kernel runs and weights, per-image tables, descriptors,
+// tile lists) never change while a kernel runs.  Reading them through the
+// constant address space lets the compiler keep wave-uniform reads on the
+// scalar unit even after the kernel's own global stores and atomics.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *as_const(const T *p)
+{
+    return (const __attribute__((address_space(4))) T *)p;
+}
+// The same for HIP's vector types, whose constructors take generic references.
+__device__ __forceinline__ int4 load_const(const int4 *p)
+{
+    const auto q = as_const((const int *)p);
+    return make_int4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ int2 load_const(const int2 *p)
+{
+    const auto q = as_const((const int *)p);
+    return make_int2(q[0], q[1]);
+}
+__device__ __forceinline__ uint2 load_const(const uint2 *p)
+{
+    const auto q = as_const((const unsigned *)p);
+    return make_uint2(q[0], q[1]);
+}
+// A descriptor struct, word by word.
+template <typename T>
+__device__ __forceinline__ T load_const_struct(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "descriptor size must be whole words");
+    const auto q = as_const((const unsigned *)p);
+    unsigned w[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = q[i];
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+
 // Separately rounded multiply and add: the same bits as the CPU path.
 __device__ __forceinline__ float mul_add_rounded(float acc, float a, float b)
 {
@@ -63,7 +102,7 @@ __device__ __forceinline__ void run_taps(const float *__restrict__ src, const fl
     if constexpr (WEIGHTED) {
         float c[L];
 #pragma unroll
-        for (int j = 0; j < L; ++j) c[j] = w[j];
+        for (int j = 0; j < L; ++j) c[j] = as_const(w)[j];
 #pragma unroll
         for (int j = 0; j < L; ++j)
 #pragma unroll
@@ -91,7 +130,8 @@ __device__ __forceinline__ void run_window(const float *__restrict__ src, const 
     for (; j + 4 <= len; j += 4) {
 #pragma unroll
         for (int q = P - 1; q < P + 3; ++q) v[q] = src[j + q];
-        const float w0 = w[j], w1 = w[j + 1], w2 = w[j + 2], w3 = w[j + 3];
+        const auto cw = as_const(w);
+        const float w0 = cw[j], w1 = cw[j + 1], w2 = cw[j + 2], w3 = cw[j + 3];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             acc[p] = tap<EXACT>(acc[p], w0, v[p]);
@@ -104,7 +144,7 @@ __device__ __forceinline__ void run_window(const float *__restrict__ src, const 
     }
     for (; j < len; ++j) {
         v[P - 1] = src[j + P - 1];
-        const float wj = w[j];
+        const float wj = as_const(w)[j];
 #pragma unroll
         for (int p = 0; p < P; ++p) acc[p] = tap<EXACT>(acc[p], wj, v[p]);
 #pragma unroll
@@ -129,7 +169,7 @@ __device__ __forceinline__ void correlate_runs(const float *__restrict__ lane, i
                                                const float *__restrict__ weights, int2 kr, float (&acc)[P])
 {
     for (int r = 0; r < kr.y; ++r) {
-        const int4 run = runs[kr.x + r];
+        const int4 run = load_const(runs + kr.x + r);
         const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
         const float *__restrict__ src = lane + off;
         const float *__restrict__ w = weights + run.w;
@@ -157,7 +197,7 @@ template <int P, bool EXACT, int BODY = 1>
 __device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp, const DevKset &ks, int k,
                                           float (&acc)[P])
 {
-    correlate_runs<P, EXACT, BODY, true>(lane, lp, ks.runs, ks.weights, ks.kern[k], acc);
+    correlate_runs<P, EXACT, BODY, true>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
 }
 
 // Stage the (TW + 2R) x (TH + 2R) window whose top-left image pixel is
@@ -310,7 +350,7 @@ __device__ __forceinline__ void prefix_runs(const uint32_t *__restrict__ lane, i
                                             int2 kr, uint32_t (&rs)[P])
 {
     for (int r = 0; r < kr.y; ++r) {
-        const int4 run = runs[kr.x + r];
+        const int4 run = load_const(runs + kr.x + r);
         const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
         const uint32_t *__restrict__ src = lane + off;
         uint32_t t[P];
@@ -345,7 +385,7 @@ __device__ __forceinline__ void quantized_runs(const float *__restrict__ lane, i
                                                int2 kr, float qscale, uint32_t (&rq)[P])
 {
     for (int r = 0; r < kr.y; ++r) {
-        const int4 run = runs[kr.x + r];
+        const int4 run = load_const(runs + kr.x + r);
         const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
         const float *__restrict__ src = lane + off;
         switch (run.z) {
