@@ -596,7 +596,7 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     const size_t off_dxt = off_tiles + up(sizeof(uint2) * tiles.size());
     const size_t off_q = off_dxt + up(sizeof(uint2) * dx_tiles.size());
     const size_t off_part = off_q + up(sizeof(float) * qpx);
-    const size_t bytes = off_part + sizeof(float) * ((size_t)grid_prep * (NW + K) + 1 + 2 * (size_t)K);
+    const size_t bytes = off_part + sizeof(float) * ((size_t)grid_prep * (NW + K) + 1 + 2 * (size_t)(NW + K));
     const size_t host_bytes = off_q;
     std::vector<unsigned char> host(host_bytes, 0);
     void *mem = nullptr;

@@ -415,35 +415,46 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
     }
 }
 
-// Pass 3a: E_k = sum of the rows' E entries.  One thread per kernel.  The
-// rows are summed in double: dL/dw = A - 2 w E / sumsq is a difference of
-// two terms that nearly cancel for some taps, so both are carried to the
-// final subtraction with no more rounding than the per-row sums hold.
-__global__ __launch_bounds__(kThreads) void reduce_e_kernel(const float *__restrict__ partial, int rows, int NW,
-                                                            int K, double *__restrict__ e_out)
+// Pass 3a: column sums of the partial rows, in double:
+// sums[c] = sum_b partial[b][c] for the NW tap columns (A) and the K kernel
+// columns (E).  dL/dw = A - 2 w E / sumsq is a difference of two terms that
+// nearly cancel for some taps, so both reach the final subtraction with no
+// more rounding than the per-row float sums hold.  64 columns per work-group
+// (consecutive lanes read consecutive columns), 16 row groups summed in a
+// fixed order.
+constexpr int kSumCols = 64, kSumRowGroups = 16;
+__global__ __launch_bounds__(kSumCols * kSumRowGroups) void column_sums_kernel(const float *__restrict__ partial,
+                                                                               int rows, int cols,
+                                                                               double *__restrict__ sums)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
+    __shared__ double part[kSumRowGroups][kSumCols];
+    const int tx = threadIdx.x % kSumCols, ty = threadIdx.x / kSumCols;
+    const int c = blockIdx.x * kSumCols + tx;
     double s = 0.0;
-    for (int b = 0; b < rows; ++b) s += (double)partial[(size_t)b * (NW + K) + NW + k];
-    e_out[k] = s;
+    if (c < cols)
+        for (int b = ty; b < rows; b += kSumRowGroups) s += (double)partial[(size_t)b * cols + c];
+    part[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && c < cols) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < kSumRowGroups; ++i) t += part[i][tx];
+        sums[c] = t;
+    }
 }
 
-// Pass 3b: dkernel[i] += sum_b A_b[i] - 2 w[i] E_k / sumsq_k, in double until
-// the one rounding to float at the end.
-__global__ __launch_bounds__(kThreads) void reduce_a_kernel(const float *__restrict__ partial, int rows, int NW,
-                                                            int K, const double *__restrict__ e_k,
-                                                            const float *__restrict__ weights,
-                                                            const int *__restrict__ tap_kernel,
-                                                            const float *__restrict__ sumsq,
-                                                            float *__restrict__ dkernel)
+// Pass 3b: dkernel[i] += A[i] - 2 w[i] E_k / sumsq_k, in double until the one
+// rounding to float at the end.
+__global__ __launch_bounds__(kThreads) void combine_kernel(const double *__restrict__ sums, int NW,
+                                                           const float *__restrict__ weights,
+                                                           const int *__restrict__ tap_kernel,
+                                                           const float *__restrict__ sumsq,
+                                                           float *__restrict__ dkernel)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= NW) return;
-    double s = 0.0;
-    for (int b = 0; b < rows; ++b) s += (double)partial[(size_t)b * (NW + K) + i];
     const int k = tap_kernel[i];
-    dkernel[i] = (float)((double)dkernel[i] + (s - 2.0 * (double)weights[i] * e_k[k] / (double)sumsq[k]));
+    dkernel[i] = (float)((double)dkernel[i] + (sums[i] - 2.0 * (double)weights[i] * sums[NW + k] / (double)sumsq[k]));
 }
 
 size_t grad_lds_bytes(int R, int K) { return sizeof(float) * ((size_t)grad_tab_offset(R) + 3 * K); }
@@ -491,12 +502,13 @@ hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAd
     hipLaunchKernelGGL(bwd_dx_kernel, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, adj, d_imgs,
                        dx_tiles, num_dx_tiles, d_counter, radius);
     if (dkernel) {
-        // K doubles after the rows, at the next 8-byte boundary
-        double *e_k = (double *)(partial + ((row_floats + 1) & ~(size_t)1));
-        hipLaunchKernelGGL(reduce_e_kernel, dim3((K + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, partial,
-                           rows, NW, K, e_k);
-        hipLaunchKernelGGL(reduce_a_kernel, dim3((NW + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, partial,
-                           rows, NW, K, e_k, ks.weights, adj.tap_kernel, adj.sumsq, dkernel);
+        // NW + K doubles after the rows, at the next 8-byte boundary
+        double *sums = (double *)(partial + ((row_floats + 1) & ~(size_t)1));
+        const int cols = NW + K;
+        hipLaunchKernelGGL(column_sums_kernel, dim3((cols + kSumCols - 1) / kSumCols), dim3(kSumCols * kSumRowGroups),
+                           0, stream, partial, rows, cols, sums);
+        hipLaunchKernelGGL(combine_kernel, dim3((NW + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, sums, NW,
+                           ks.weights, adj.tap_kernel, adj.sumsq, dkernel);
     }
     return hipGetLastError();
 }

@@ -122,7 +122,7 @@ int blocks_per_cu(int shape, int radius);
 // Adjoint (lpa_bwd.hip).  The first pass walks 64 x 32 tiles (d_tiles), the
 // dx pass 64 x 64 tiles (dx_tiles); both queues are laid out as for the
 // batched forward ({image, (tile_y << 16) | tile_x}).
-//   partial: [grid_prep][num_weights + K] floats (+1+2K for the E sums in double), one row
+//   partial: [grid_prep][num_weights + K] floats (+1+2(num_weights+K) for the column sums in double), one row
 //            per work-group of the first pass, accumulated by its waves
 //   dkernel: += reduction of the rows plus the variance term (may be null)
 constexpr int kBwdTileW = 64, kBwdTileH = 32, kDxTile = 64;

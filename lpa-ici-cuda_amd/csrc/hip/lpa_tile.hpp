@@ -316,54 +316,49 @@ __device__ __forceinline__ bool stage_prefix(uint32_t *__restrict__ q, int lp, c
     return block_all(fits);
 }
 
-// t[p] = sum of a run of length L for the P pixels whose left-boundary
-// prefix values start at src: t[p] = src[L + p] - src[p].
-template <int L, int P>
-__device__ __forceinline__ void prefix_run(const uint32_t *__restrict__ src, uint32_t (&t)[P])
-{
-    uint32_t v[L + P];
-#pragma unroll
-    for (int i = 0; i < L + P; ++i) v[i] = src[i];
-#pragma unroll
-    for (int p = 0; p < P; ++p) t[p] = v[L + p] - v[p];
-}
-template <int P>
-__device__ __forceinline__ void prefix_run_long(const uint32_t *__restrict__ src, int len, uint32_t (&t)[P])
-{
-    uint32_t a[P], b[P];
-    const uint32_t *__restrict__ e = src + len;
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        a[p] = src[p];
-        b[p] = e[p];
-    }
-#pragma unroll
-    for (int p = 0; p < P; ++p) t[p] = b[p] - a[p];
-}
-
 // rs[p] += fixed-point sum over the runs kr of the tile, for the P pixels
 // whose left-boundary prefix values start at `lane` (prefix table as written
-// by stage_prefix).  A run of length L <= P reads L + P prefix values, a
-// longer one 2P; either way two integer adds per pixel.
+// by stage_prefix): a run (row dr, column c0, length L) adds
+// Q[dr][c0 + L + p] - Q[dr][c0 + p], whatever its length.  Runs go in pairs:
+// one scalar load brings both records, the four windows of P prefix values
+// are read together (4P reads in flight), and the pair costs about three
+// integer operations per pixel.  There is no branch on the run length.
 template <int P>
 __device__ __forceinline__ void prefix_runs(const uint32_t *__restrict__ lane, int lp, const int4 *__restrict__ runs,
                                             int2 kr, uint32_t (&rs)[P])
 {
-    for (int r = 0; r < kr.y; ++r) {
-        const int4 run = load_const(runs + kr.x + r);
-        const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
-        const uint32_t *__restrict__ src = lane + off;
-        uint32_t t[P];
-        switch (run.z) {
-#define LPA_PREFIX_CASE(L)                                                                                         \
-    case L: prefix_run<L, P>(src, t); break;
-            LPA_PREFIX_CASE(1) LPA_PREFIX_CASE(2) LPA_PREFIX_CASE(3) LPA_PREFIX_CASE(4)
-            LPA_PREFIX_CASE(5) LPA_PREFIX_CASE(6) LPA_PREFIX_CASE(7) LPA_PREFIX_CASE(8)
-#undef LPA_PREFIX_CASE
-        default: prefix_run_long<P>(src, run.z, t); break;
+    int r = 0;
+    for (; r + 1 < kr.y; r += 2) {
+        const int4 u = load_const(runs + kr.x + r), v = load_const(runs + kr.x + r + 1);
+        const int ou = u.x * lp + u.y, ov = v.x * lp + v.y;
+        const uint32_t *__restrict__ ua = lane + ou;
+        const uint32_t *__restrict__ ub = lane + (ou + u.z);
+        const uint32_t *__restrict__ va = lane + ov;
+        const uint32_t *__restrict__ vb = lane + (ov + v.z);
+        uint32_t a0[P], b0[P], a1[P], b1[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            a0[p] = ua[p];
+            b0[p] = ub[p];
+            a1[p] = va[p];
+            b1[p] = vb[p];
         }
 #pragma unroll
-        for (int p = 0; p < P; ++p) rs[p] += t[p];
+        for (int p = 0; p < P; ++p) rs[p] = (rs[p] + b0[p] + b1[p]) - (a0[p] + a1[p]);
+    }
+    if (r < kr.y) {
+        const int4 u = load_const(runs + kr.x + r);
+        const int ou = u.x * lp + u.y;
+        const uint32_t *__restrict__ ua = lane + ou;
+        const uint32_t *__restrict__ ub = lane + (ou + u.z);
+        uint32_t a0[P], b0[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            a0[p] = ua[p];
+            b0[p] = ub[p];
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) rs[p] += b0[p] - a0[p];
     }
 }
 
