@@ -47,10 +47,12 @@ struct lpa_kset {
     std::vector<int4> ring_runs;
     std::vector<int2> ring_kern;
     std::vector<float> ring_scale;
+    std::vector<int4> ring_runs_flip;  // the rings rotated by 180 degrees (adjoint)
     int ring_taps = 0;
     int ring_frac_bits = 0;      // F: ring sums take tile values as rint(v * 2^F)
     float ring_range = 0.0f;     // fixed point for windows with every |v| below this
     int4 *d_ring_runs = nullptr;
+    int4 *d_ring_runs_flip = nullptr;
     int2 *d_ring_kern = nullptr;
     float *d_ring_scale = nullptr;
     bool uploaded = false;
@@ -59,6 +61,7 @@ struct lpa_kset {
     lpa::DevRings rings() const
     {
         return lpa::DevRings{d_ring_runs,
+                             d_ring_runs_flip,
                              d_ring_kern,
                              d_ring_scale,
                              std::ldexp(1.0f, ring_frac_bits),
@@ -212,6 +215,7 @@ void build_rings(lpa_kset *ks, const float *dense, const int *offsets, const int
             ks->ring_kern.push_back(int2{first_run, (int)ks->ring_runs.size() - first_run});
             ks->ring_taps += (int)ring.size();
         }
+    for (const int4 &r : ks->ring_runs) ks->ring_runs_flip.push_back(int4{-r.x, -(r.y + r.z - 1), r.z, 0});
     // Fixed point for ring sums: with n_max the largest support, F is the
     // largest power making n_max * 2 * 2^F < 2^31, and the range bound r
     // keeps every sum exact: |rint(v 2^F)| < r 2^F + 1/2 for |v| < r, so a
@@ -313,6 +317,10 @@ lpa_status ensure_uploaded(lpa_kset *ks)
     if (e == hipSuccess && ks->has_rings) {
         const size_t nrr = std::max<size_t>(1, ks->ring_runs.size());
         e = hipMalloc(&ks->d_ring_runs, sizeof(int4) * nrr);
+        if (e == hipSuccess) e = hipMalloc(&ks->d_ring_runs_flip, sizeof(int4) * nrr);
+        if (e == hipSuccess && !ks->ring_runs_flip.empty())
+            e = hipMemcpy(ks->d_ring_runs_flip, ks->ring_runs_flip.data(), sizeof(int4) * ks->ring_runs_flip.size(),
+                          hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc(&ks->d_ring_kern, sizeof(int2) * K);
         if (e == hipSuccess) e = hipMalloc(&ks->d_ring_scale, sizeof(float) * K);
         if (e == hipSuccess && !ks->ring_runs.empty())
@@ -472,7 +480,7 @@ void lpa_kset_destroy(lpa_kset_t ks)
     if (ks->d_kern) (void)hipFree(ks->d_kern);
     for (void *p : {(void *)ks->d_runs_flip, (void *)ks->d_weights_flip, (void *)ks->d_tap_pos,
                     (void *)ks->d_tap_kernel, (void *)ks->d_kern_w, (void *)ks->d_sumsq, (void *)ks->d_ring_runs,
-                    (void *)ks->d_ring_kern, (void *)ks->d_ring_scale})
+                    (void *)ks->d_ring_runs_flip, (void *)ks->d_ring_kern, (void *)ks->d_ring_scale})
         if (p) (void)hipFree(p);
     delete ks;
 }

@@ -17,7 +17,8 @@
 // wave's A partials are summed across its lanes by a reduce-scatter and
 // added into the wave's own row of `partial`.
 // Pass 2 (dx): per direction and scale, the masked map q*inv*[s_d = s] over
-// tile + halo is correlated with the 180-degree rotated kernel.
+// tile + halo is correlated with the 180-degree rotated kernel (for ring-form
+// sets, nested masked maps with the rotated rings).
 // Pass 3: rows of `partial` are summed and the variance term is added.
 #include <hip/hip_runtime.h>
 
@@ -334,9 +335,18 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
 constexpr int DXW = 64, DXH = 64, DXP = 4, DXT = 1024, DXLX = DXW / DXP;
 constexpr int kSlotRows = 8, kSlotCols = 2;  // 16 + 16i rows, 64 + 64j columns cover 126 x 126
 
-__global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj, const BwdDesc *__restrict__ imgs,
-                                                     const uint2 *__restrict__ tiles, int num_tiles,
-                                                     int *__restrict__ counter, int R)
+//
+// RING (kernel sets in ring form, k_{d,s} = w_{d,s} on support(d,s), supports
+// nested): with M_{d,r}(p) = q(p) inv_{d,s} w_{d,s} [s = s_d(p) >= r],
+//   dL/dx[u] = sum_d sum_r sum_{t in ring r} M_{d,r}(u - t),
+// since support(d,s) is the union of the rings r <= s.  Walking r from S-1
+// down to 0 the map only gains the pixels with s_d(p) = r, and each
+// 180-degree-rotated ring is applied once per direction: float adds over the
+// ring taps instead of FMAs over every tap of every scale present.
+template <bool RING>
+__global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
+                                                     const BwdDesc *__restrict__ imgs, const uint2 *__restrict__ tiles,
+                                                     int num_tiles, int *__restrict__ counter, int R)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int next;
@@ -388,23 +398,45 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevAdjoint adj,
                     const bool in = r < fh && c < fw && gy >= 0 && gy < img.H && gx >= 0 && gx < img.W;
                     sv[i][j] = in ? as_global(img.map)[(size_t)d * plane + (size_t)gy * img.W + gx] : 255;
                 }
-            for (int s = 0; s < S; ++s) {
-                int any = 0;
-                __syncthreads();  // previous correlation done with the map
-                const float inv = as_const(inv_t)[d * S + s];
+            if constexpr (RING) {
+                for (int s = S - 1; s >= 0; --s) {
+                    bool any = false;
+                    __syncthreads();  // previous ring sums done with the map
+                    const int k = d * S + s;
+                    const float cw = as_const(inv_t)[k] * as_const(rings.scale)[k];
 #pragma unroll
-                for (int i = 0; i < kSlotRows; ++i)
+                    for (int i = 0; i < kSlotRows; ++i)
 #pragma unroll
-                    for (int j = 0; j < kSlotCols; ++j) {
-                        const int r = r0 + 16 * i, c = c0 + 64 * j;
-                        if (r < fh && c < fw) {
-                            const bool m = sv[i][j] == s && qv[i][j] != 0.0f;
-                            any |= m;
-                            mt[r * lp + c] = m ? qv[i][j] * inv : 0.0f;
+                        for (int j = 0; j < kSlotCols; ++j) {
+                            const int r = r0 + 16 * i, c = c0 + 64 * j;
+                            if (r < fh && c < fw) {
+                                // the first (largest) scale writes every element
+                                if (s == S - 1 || sv[i][j] == s) mt[r * lp + c] = sv[i][j] == s ? qv[i][j] * cw : 0.0f;
+                                any |= sv[i][j] >= s && sv[i][j] < S && qv[i][j] != 0.0f;
+                            }
                         }
-                    }
-                if (!__syncthreads_or(any)) continue;
-                correlate<DXP, false, 0>(lane_m, lp, adj.flip, d * S + s, dx);
+                    if (!block_vote<DXT, false>(any)) continue;
+                    correlate_runs<DXP, false, 1, false>(lane_m, lp, rings.flip, nullptr, load_const(rings.kern + k), dx);
+                }
+            } else {
+                for (int s = 0; s < S; ++s) {
+                    bool any = false;
+                    __syncthreads();  // previous correlation done with the map
+                    const float inv = as_const(inv_t)[d * S + s];
+#pragma unroll
+                    for (int i = 0; i < kSlotRows; ++i)
+#pragma unroll
+                        for (int j = 0; j < kSlotCols; ++j) {
+                            const int r = r0 + 16 * i, c = c0 + 64 * j;
+                            if (r < fh && c < fw) {
+                                const bool m = sv[i][j] == s && qv[i][j] != 0.0f;
+                                any |= m;
+                                mt[r * lp + c] = m ? qv[i][j] * inv : 0.0f;
+                            }
+                        }
+                    if (!block_vote<DXT, false>(any)) continue;
+                    correlate<DXP, false, 0>(lane_m, lp, adj.flip, d * S + s, dx);
+                }
             }
         }
         const auto out = as_global(bd.dx) + (size_t)py * img.pitch + px0;
@@ -475,7 +507,7 @@ void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, 
         b = 1;
     *prep = b > 0 ? b : 1;
     b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_dx_kernel, DXT, dx_lds_bytes(radius, num_kernels)) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_dx_kernel<true>, DXT, dx_lds_bytes(radius, num_kernels)) !=
         hipSuccess)
         b = 1;
     *dx = b > 0 ? b : 1;
@@ -499,8 +531,12 @@ hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAd
     else
         hipLaunchKernelGGL(bwd_grad_kernel<false>, dim3(grid_prep), dim3(kThreads), grad_lds_bytes(radius, K), stream,
                            ks, rings, adj, d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
-    hipLaunchKernelGGL(bwd_dx_kernel, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, adj, d_imgs,
-                       dx_tiles, num_dx_tiles, d_counter, radius);
+    if (rings.runs)
+        hipLaunchKernelGGL(bwd_dx_kernel<true>, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, rings,
+                           adj, d_imgs, dx_tiles, num_dx_tiles, d_counter, radius);
+    else
+        hipLaunchKernelGGL(bwd_dx_kernel<false>, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, rings,
+                           adj, d_imgs, dx_tiles, num_dx_tiles, d_counter, radius);
     if (dkernel) {
         // NW + K doubles after the rows, at the next 8-byte boundary
         double *sums = (double *)(partial + ((row_floats + 1) & ~(size_t)1));

@@ -30,6 +30,7 @@ struct DevKset {
 // scale's estimate is scale[k] times the running sum over its rings.
 struct DevRings {
     const int4 *runs;
+    const int4 *flip;  // the ring runs rotated by 180 degrees (same kern ranges)
     const int2 *kern;
     const float *scale;
     float qscale;  // 2^F: fixed-point scale of tile values for ring sums

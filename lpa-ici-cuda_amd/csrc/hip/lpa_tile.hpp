@@ -270,17 +270,19 @@ __device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
 // __syncthreads_and would act as a memory clobber for the compiler, and every
 // kernel-table load after it would become a vector load.  The vote slots are
 // rewritten by the next call only after the caller's next barrier.
-__device__ __forceinline__ bool block_all(bool v)
+template <int NT, bool ALL>
+__device__ __forceinline__ bool block_vote(bool v)
 {
-    __shared__ int votes[kThreads / 64];
-    const bool w = __all(v);
+    __shared__ int votes[NT / 64];
+    const bool w = ALL ? __all(v) : __any(v);
     if ((threadIdx.x & 63) == 0) votes[threadIdx.x >> 6] = w;
     __syncthreads();
-    int r = 1;
+    int r = ALL ? 1 : 0;
 #pragma unroll
-    for (int i = 0; i < kThreads / 64; ++i) r &= votes[i];
+    for (int i = 0; i < NT / 64; ++i) r = ALL ? (r & votes[i]) : (r | votes[i]);
     return __builtin_amdgcn_readfirstlane(r) != 0;
 }
+__device__ __forceinline__ bool block_all(bool v) { return block_vote<kThreads, true>(v); }
 
 // Stage the (TW + 2R) x (TH + 2R) window (as stage_tile) as row-prefix sums
 // of fixed-point values: q[r * lp + j] = Q[r][j] for j = 0 .. TW + 2R
