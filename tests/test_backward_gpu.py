@@ -4,6 +4,14 @@ the FP64 CPU adjoint and FP64 central differences.
 Bar: gradients within 1e-3 relative of FP64 central differences (eps 1e-3)
 at perturbations that flip no selection of the FP32 walk; against the exact
 FP64 adjoint the FP32 kernels are held to 1e-4 of the gradient's scale.
+
+The relative test has a floor proportional to the gradient's scale,
+|g - fd| <= 1e-3 * max(|fd|, 1e-2 * max|g|): for components far below the
+scale, neither side is accurate to 1e-3 of the component itself.  The
+central differences carry O(eps^2) truncation (up to 6e-4 against the exact
+FP64 adjoint on taps of gradient ~5, about 1e-4 relative), and FP32 sums are
+accurate to ~1e-6 of the scale.  Formula errors (a wrong sign, a missing
+term, a wrong tap) are orders of magnitude above this floor.
 """
 import numpy as np
 import pytest
@@ -93,7 +101,7 @@ def test_adjoint_central_differences_image(ks):
         # the float32 inputs are not exactly x +- EPS: divide by the true step
         step = float(np.float64(xp[r, c]) - np.float64(xm[r, c]))
         fd = (loss64(xp, gh, ks, maps) - loss64(xm, gh, ks, maps)) / step
-        assert abs(dx[r, c] - fd) <= TOL_FD * max(abs(fd), 1e-2), (r, c, dx[r, c], fd)
+        assert abs(dx[r, c] - fd) <= TOL_FD * max(abs(fd), 1e-2 * np.abs(dx).max()), (r, c, dx[r, c], fd)
         checked += 1
     assert checked >= 32
 
@@ -122,7 +130,7 @@ def test_adjoint_central_differences_taps(ks):
             continue
         step = float(np.float64(kp.weights[di]) - np.float64(km.weights[di]))
         fd = (loss64(x, gh, kp, maps) - loss64(x, gh, km, maps)) / step
-        assert abs(dk[i] - fd) <= TOL_FD * max(abs(fd), 1e-2), (int(i), k, dk[i], fd)
+        assert abs(dk[i] - fd) <= TOL_FD * max(abs(fd), 1e-2 * np.abs(dk).max()), (int(i), k, dk[i], fd)
         checked += 1
         if checked == 64:
             break
