@@ -159,6 +159,19 @@ lpa_status lpa_tile_config_get(int index, lpa_tile_config* cfg);
 lpa_status lpa_kset_set_tile_config(lpa_kset_t ks, int index);
 int lpa_kset_get_tile_config(lpa_kset_t ks);
 
+/* Per-direction estimates at the scales a forward selected: for each image,
+ * est[i] (device, [D][height][width] floats, dense) receives, per direction
+ * d and pixel p, the estimate of d at scale scale_map[d][p] (the map is
+ * required), computed with the forward's walk for the same flags
+ * (LPA_FWD_EXACT, LPA_FWD_WEIGHTED, LPA_FWD_RING_FLOAT), so it is the value
+ * that forward aggregated, bit for bit.  These are the reference's per
+ * direction debugging outputs (the commented-out writes of img_denoised_d and
+ * scale_map_d, src/v0_0_lpa_ici_2D_naive.cpp:848, 901, 906-914).  Errors:
+ * LPA_EINVAL (missing scale map or est, LPA_FWD_GRAPH), LPA_ETOOBIG, LPA_ENOMEM,
+ * LPA_EHIP. */
+lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image* imgs, float* const* est, int n, unsigned flags,
+                             void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Adjoint (selection frozen at the forward's scale maps)                   */
 /* ---------------------------------------------------------------------- */

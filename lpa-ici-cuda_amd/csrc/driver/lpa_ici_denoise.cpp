@@ -69,6 +69,10 @@ int main(int argc, char **argv)
     if (lpa_write_hdr(cfg.output_file.c_str(), width, height, denoised.data()) != LPA_OK)
         std::cerr << "Error: " << lpa_last_error() << "\n";
 
+    if (!cfg.debug_dir.empty())
+        write_direction_outputs(noisy.data(), width, height, num_dirs, num_scales, weights.data(), offsets.data(),
+                                rows.data(), cols.data(), variances.data(), cfg.ici_gamma, cfg.debug_dir);
+
     std::printf("{\"width\": %d, \"height\": %d, \"psnr_noisy\": %.10f, \"psnr_denoised\": %.10f, "
                 "\"seconds\": %.6f}\n",
                 width, height, psnr_noisy, psnr_denoised, seconds);

@@ -562,6 +562,65 @@ lpa_status lpa_ici_forward(lpa_kset_t ks, const lpa_image *imgs, int n, unsigned
     return st;
 }
 
+lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const *est, int n, unsigned flags,
+                             void *stream)
+{
+    lpa_status st = check_images(ks, imgs, n);
+    if (st != LPA_OK) return st;
+    if ((st = ensure_uploaded(ks)) != LPA_OK) return st;
+    if (flags & LPA_FWD_GRAPH) return fail(LPA_EINVAL, "LPA_FWD_GRAPH applies to plans only");
+    if (!est) return fail(LPA_EINVAL, "null est array");
+    for (int i = 0; i < n; ++i) {
+        if (!imgs[i].scale_map)
+            return fail(LPA_EINVAL, "image " + std::to_string(i) + ": the scale map of a forward is required");
+        if (!est[i]) return fail(LPA_EINVAL, "image " + std::to_string(i) + ": null est");
+    }
+    hipStream_t s = (hipStream_t)stream;
+    constexpr int SH = lpa::kEstimateShape;
+    std::vector<uint2> tiles;
+    for (int i = 0; i < n; ++i) {
+        const int tx = tiles_x_of(SH, imgs[i].width), ty = tiles_y_of(SH, imgs[i].height);
+        for (int y = 0; y < ty; ++y)
+            for (int x = 0; x < tx; ++x) tiles.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
+    }
+    const int grid = std::max(1, std::min((int)tiles.size(), num_cus() * lpa::blocks_per_cu(SH, ks->radius)));
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t off_desc = 256;
+    const size_t off_tiles = off_desc + up(sizeof(lpa::EstDesc) * n);
+    const size_t bytes = off_tiles + sizeof(uint2) * tiles.size();
+    std::vector<unsigned char> host(bytes, 0);
+    std::vector<lpa::EstDesc> desc(n);
+    for (int i = 0; i < n; ++i) {
+        const lpa_image &m = imgs[i];
+        lpa::EstDesc &e = desc[i];
+        e.img.x = m.x;
+        e.img.y = m.y;
+        e.img.map = m.scale_map;
+        e.img.tab = nullptr;
+        e.img.W = m.width;
+        e.img.H = m.height;
+        e.img.pitch = m.pitch ? m.pitch : m.width;
+        e.img.tiles_x = tiles_x_of(SH, m.width);
+        e.est = est[i];
+    }
+    std::memcpy(host.data() + off_desc, desc.data(), sizeof(lpa::EstDesc) * n);
+    std::memcpy(host.data() + off_tiles, tiles.data(), sizeof(uint2) * tiles.size());
+    void *mem = nullptr;
+    HIP_TRY(hipMallocAsync(&mem, bytes, s));
+    unsigned char *base = (unsigned char *)mem;
+    hipError_t e = hipMemcpyAsync(mem, host.data(), bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
+    if (e == hipSuccess)
+        e = lpa::launch_estimates(walk_mode(ks, flags), ks->dev(), ks->has_rings ? ks->rings() : lpa::DevRings{},
+                                  (const lpa::EstDesc *)(base + off_desc), (const uint2 *)(base + off_tiles),
+                                  (int)tiles.size(), (int *)base, ks->radius, grid, s);
+    hipError_t ef = hipFreeAsync(mem, s);
+    if (e != hipSuccess) return hip_fail(e, "estimates");
+    if (ef != hipSuccess) return hip_fail(ef, "hipFreeAsync");
+    lpa::clear_error();
+    return LPA_OK;
+}
+
 lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *const *dy, float *const *dx,
                             float *dkernel, int n, void *stream)
 {

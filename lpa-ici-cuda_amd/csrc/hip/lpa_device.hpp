@@ -120,6 +120,18 @@ hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, c
 // Work-groups of one shape that fit on a CU at this radius (LDS bound).
 int blocks_per_cu(int shape, int radius);
 
+// Per-direction estimates at frozen scales (lpa_fwd.hip): est = [D][H][W]
+// floats per image, dense like the scale map.  Tile configuration
+// kEstimateShape, a persistent grid over the tile queue.
+struct EstDesc {
+    ImgDesc img;
+    float *est;
+};
+constexpr int kEstimateShape = 5;
+hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &rings, const EstDesc *d_imgs,
+                            const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
+                            hipStream_t stream);
+
 // Adjoint (lpa_bwd.hip).  The first pass walks 64 x 32 tiles (d_tiles), the
 // dx pass 64 x 64 tiles (dx_tiles); both queues are laid out as for the
 // batched forward ({image, (tile_y << 16) | tile_x}).

@@ -171,6 +171,119 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
     }
 }
 
+// Per-direction estimates at frozen scales (the scale maps a forward wrote):
+// est[d][p] = the estimate of direction d at scale s_d(p), from the same walk
+// and arithmetic as the forward with the same walk mode, so it is the value
+// the forward aggregated, bit for bit.  Per lane, scales are walked up to the
+// deepest one its pixels selected (ring forms: running sums; weighted walks:
+// only the selected kernels).
+template <int P, int MODE, int BODY, typename T>
+__device__ __forceinline__ void estimate_lane(const T *__restrict__ src, int lp, const DevKset &ks,
+                                              const DevRings &rings, const uint8_t *__restrict__ map,
+                                              float *__restrict__ est, int W, int H, int px0, int py)
+{
+    constexpr bool EXACT = MODE == kWalkExact;
+    const int S = ks.S;
+    const size_t plane = (size_t)W * H;
+    const size_t pix = (size_t)py * W + px0;
+    bool inside[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) inside[p] = py < H && px0 + p < W;
+    for (int d = 0; d < ks.D; ++d) {
+        const auto gmap = as_global(map) + (size_t)d * plane + pix;
+        int sd[P];
+        float out[P], ring_sum[P];
+        uint32_t ring_q[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            sd[p] = inside[p] ? min((int)gmap[p], S - 1) : S;
+            out[p] = 0.0f;
+            ring_sum[p] = 0.0f;
+            ring_q[p] = 0u;
+        }
+        for (int s = 0; s < S; ++s) {
+            bool need = false, here = false;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                need |= sd[p] >= s && sd[p] < S;
+                here |= sd[p] == s;
+            }
+            if (!__any(need)) break;
+            const int k = d * S + s;
+            float acc[P];
+            if constexpr (MODE == kWalkRing) {
+                if (need) prefix_runs<P>(src, lp, rings.runs, load_const(rings.kern + k), ring_q);
+                const float wq = as_const(rings.scale)[k] * rings.qinv;
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
+            } else if constexpr (MODE == kWalkRingFloat) {
+                if (need)
+                    correlate_runs<P, false, 1, false>(src, lp, rings.runs, nullptr, load_const(rings.kern + k),
+                                                       ring_sum);
+                const float w = as_const(rings.scale)[k];
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;
+            } else {
+                if (!__any(here)) continue;
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p] = 0.0f;
+                if (here) correlate<P, EXACT, BODY>(src, lp, ks, k, acc);
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+                if (sd[p] == s) out[p] = acc[p];
+        }
+        const auto gest = as_global(est) + (size_t)d * plane + pix;
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+            if (inside[p]) gest[p] = out[p];
+    }
+}
+
+// Persistent estimates kernel over a tile queue spanning a batch, on tile
+// configuration SH's shape.
+template <int SH, int MODE>
+__global__ __launch_bounds__(kThreads) void estimate_kernel(DevKset ks, DevRings rings,
+                                                            const EstDesc *__restrict__ imgs,
+                                                            const uint2 *__restrict__ tiles, int num_tiles,
+                                                            int *__restrict__ counter, int R)
+{
+    extern __shared__ float tile[];
+    __shared__ int next;
+    constexpr TileShape sh = kShapes[SH];
+    constexpr int LX = sh.tw / sh.px;
+    const int lp = lds_pitch(sh.tw + 1, R, sh.pm);
+    const int tid = threadIdx.x, tx = tid % LX, ty = tid / LX;
+    const int lane_off = (ty + R) * lp + tx * sh.px + R;
+    for (;;) {
+        if (tid == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = next;
+        if (t >= num_tiles) break;
+        const uint2 e = load_const(tiles + t);
+        const EstDesc ed = load_const_struct(imgs + e.x);
+        const ImgDesc &img = ed.img;
+        const int x0 = (int)(e.y & 0xffffu) * sh.tw, y0 = (int)(e.y >> 16) * sh.th;
+        const int px0 = x0 + tx * sh.px, py = y0 + ty;
+        bool fixed = false;
+        if constexpr (MODE == kWalkRing)
+            fixed = stage_prefix<sh.tw, sh.th>((uint32_t *)tile, lp, img.x, img.W, img.H, img.pitch, x0, y0, R,
+                                               rings.qscale, rings.qrange);
+        if (fixed) {
+            estimate_lane<sh.px, kWalkRing, sh.body>((const uint32_t *)tile + lane_off, lp, ks, rings, img.map,
+                                                      ed.est, img.W, img.H, px0, py);
+        } else {
+            __syncthreads();  // every lane has voted before the window is restaged
+            stage_tile<sh.tw, sh.th>(tile + 1, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
+            __syncthreads();
+            constexpr int M = MODE == kWalkRing ? kWalkRingFloat : MODE;
+            estimate_lane<sh.px, M, sh.body>(tile + 1 + lane_off, lp, ks, rings, img.map, ed.est, img.W, img.H, px0,
+                                              py);
+        }
+        __syncthreads();  // the window and `next` are reused by the next tile
+    }
+}
+
 // Plain form: one launch per image.  Work-groups are dealt to the 8 XCDs in
 // turn, so block b runs tile (b % 8) * per_xcd + b / 8: consecutive tiles of
 // one XCD are neighbours and share their halos in that XCD's L2.
@@ -305,6 +418,27 @@ hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, c
 #undef LPA_B
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &rings, const EstDesc *d_imgs,
+                            const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
+                            hipStream_t stream)
+{
+    constexpr int SH = kEstimateShape;
+    constexpr TileShape sh = kShapes[SH];
+    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw + 1, sh.th, radius, sh.pm);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
+    switch (mode) {
+#define LPA_E(M)                                                                                                  \
+    case M:                                                                                                        \
+        hipLaunchKernelGGL((estimate_kernel<SH, M>), dim3(grid), dim3(kThreads), lds, stream, ks, rings, d_imgs,       \
+                           d_tiles, num_tiles, d_counter, radius);                                                 \
+        break;
+        LPA_E(kWalkFma) LPA_E(kWalkExact) LPA_E(kWalkRing) LPA_E(kWalkRingFloat)
+#undef LPA_E
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 int blocks_per_cu(int shape, int radius)

@@ -14,6 +14,7 @@ struct Config {
     float sigma_noise = 20.0f / 255.0f;
     unsigned int noise_seed = 20250910u;
     float ici_gamma = 2.0f;
+    std::string debug_dir;  // non-empty: write the per-direction debugging outputs there
 };
 
 // Directory part of a path, trailing slash included ("" if none).
@@ -36,6 +37,7 @@ inline void print_usage(const char *prog)
               << "  -s, --sigma <value>    noise standard deviation on the 0-255 scale (default 20)\n"
               << "  -g, --gamma <value>    ICI threshold (default " << d.ici_gamma << ")\n"
               << "      --seed <value>     noise seed (default " << d.noise_seed << ")\n"
+              << "      --debug <dir>      also write each direction's estimate and scale map as .hdr in <dir>\n"
               << "  -h, --help             show this message\n";
 }
 
@@ -78,7 +80,9 @@ inline Config parse_args(int argc, char **argv, const std::string &version)
             const unsigned long v = std::strtoul(val, &end, 10);
             if (*val == '\0' || *end != '\0') die("--seed needs an unsigned integer, got '" + std::string(val) + "'");
             cfg.noise_seed = static_cast<unsigned int>(v);
-        } else
+        } else if (opt == "--debug")
+            cfg.debug_dir = val;
+        else
             die("unknown option '" + opt + "'");
     }
     if (cfg.output_file.empty()) {

@@ -5,8 +5,11 @@
 // device to after the result is back on the host.
 #include <hip/hip_runtime_api.h>
 
+#include <cstdint>
 #include <cstdlib>
 #include <iostream>
+#include <string>
+#include <vector>
 
 #include "lpa_ici.h"
 #include "lpa_ici_host.hpp"
@@ -76,5 +79,60 @@ void anisotropic_lpa_ici_gpu_fused(float *host_img_noisy, float *host_img_denois
     lpa_plan_destroy(plan);
     (void)hipFree(d_x);
     (void)hipFree(d_y);
+    lpa_kset_destroy(ks);
+}
+
+// The reference's per-direction debugging outputs (commented out at
+// src/v0_0_lpa_ici_2D_naive.cpp:906-914): for each direction d,
+// <dir>gray_image_denoised_theta_<d>.hdr (the estimate at the selected scale)
+// and <dir>scale_map_theta_<d>.hdr (s / (S - 1), as the reference's
+// scale_map_d, :848 and :901), from one forward that keeps the uint8 scale
+// map and lpa_ici_estimates.
+void write_direction_outputs(const float *host_img_noisy, int width, int height, int num_dirs, int num_scales,
+                             const float *host_kernel_weights, const int *host_offsets, const int *host_rows,
+                             const int *host_cols, const float *host_kernel_variances, float ici_gamma,
+                             const std::string &dir)
+{
+    const size_t npx = (size_t)width * height;
+    lpa_kset_t ks = nullptr;
+    check(lpa_kset_create(num_dirs, num_scales, host_kernel_weights, host_offsets, host_rows, host_cols, &ks),
+          "kernel set");
+    float *d_x = nullptr, *d_y = nullptr, *d_est = nullptr;
+    uint8_t *d_map = nullptr;
+    check(hipMalloc(&d_x, sizeof(float) * npx), "hipMalloc");
+    check(hipMalloc(&d_y, sizeof(float) * npx), "hipMalloc");
+    check(hipMalloc(&d_est, sizeof(float) * npx * num_dirs), "hipMalloc");
+    check(hipMalloc(&d_map, npx * num_dirs), "hipMalloc");
+    check(hipMemcpy(d_x, host_img_noisy, sizeof(float) * npx, hipMemcpyHostToDevice), "hipMemcpy");
+    lpa_image img{};
+    img.x = d_x;
+    img.y = d_y;
+    img.scale_map = d_map;
+    img.width = width;
+    img.height = height;
+    img.gamma = ici_gamma;
+    img.variances = host_kernel_variances;
+    check(lpa_ici_forward(ks, &img, 1, LPA_FWD_PLAIN, nullptr), "forward");
+    float *const est[1] = {d_est};
+    check(lpa_ici_estimates(ks, &img, est, 1, 0, nullptr), "estimates");
+    std::vector<float> h_est(npx * num_dirs), plane(npx);
+    std::vector<uint8_t> h_map(npx * num_dirs);
+    check(hipMemcpy(h_est.data(), d_est, sizeof(float) * h_est.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    check(hipMemcpy(h_map.data(), d_map, h_map.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    const std::string prefix = dir.empty() || dir.back() == '/' ? dir : dir + "/";
+    for (int d = 0; d < num_dirs; ++d) {
+        const std::string est_file = prefix + "gray_image_denoised_theta_" + std::to_string(d) + ".hdr";
+        const std::string map_file = prefix + "scale_map_theta_" + std::to_string(d) + ".hdr";
+        check(lpa_write_hdr(est_file.c_str(), width, height, h_est.data() + (size_t)d * npx), "write_hdr");
+        for (size_t i = 0; i < npx; ++i)
+            plane[i] = num_scales > 1 ? (float)h_map[(size_t)d * npx + i] / (num_scales - 1) : 0.0f;
+        check(lpa_write_hdr(map_file.c_str(), width, height, plane.data()), "write_hdr");
+    }
+    std::cout << "Saved " << 2 * num_dirs << " per-direction estimates and scale maps in " << (dir.empty() ? "." : dir)
+              << "\n";
+    (void)hipFree(d_x);
+    (void)hipFree(d_y);
+    (void)hipFree(d_est);
+    (void)hipFree(d_map);
     lpa_kset_destroy(ks);
 }

@@ -39,6 +39,15 @@ void anisotropic_lpa_ici_gpu_fused(float *host_img_noisy, float *host_img_denois
                                    int *host_offsets, int *host_rows, int *host_cols,
                                    float *host_kernel_variances, float ici_gamma);
 
+// Per-direction debugging outputs of the reference's v0_0 (commented out
+// there, src/v0_0_lpa_ici_2D_naive.cpp:906-914): the selected estimate and
+// the scale map s/(S-1) of every direction as .hdr files in `dir`.  Exits
+// with EXIT_FAILURE on any failure, like the orchestrator.
+void write_direction_outputs(const float *host_img_noisy, int width, int height, int num_dirs, int num_scales,
+                             const float *host_kernel_weights, const int *host_offsets, const int *host_rows,
+                             const int *host_cols, const float *host_kernel_variances, float ici_gamma,
+                             const std::string &dir);
+
 namespace lpa {
 
 // Read a whole text file, transparently gunzipping names ending in ".gz".

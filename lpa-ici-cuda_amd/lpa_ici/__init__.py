@@ -17,7 +17,7 @@ __all__ = [
     "DEFAULT_KERNELS", "GAUSSIAN_M20_KERNELS", "KERNEL_DIR", "LIB_PATH", "FWD_PLAIN", "FWD_EXACT", "FWD_GRAPH",
     "FWD_WEIGHTED", "FWD_RING_FLOAT",
     "LpaError", "KernelSet", "Plan", "load_kernels", "kernel_variances", "make_noisy", "psnr", "synth_image",
-    "load_image", "write_hdr", "denoise", "backward", "image_struct", "tile_configs", "lib",
+    "load_image", "write_hdr", "denoise", "estimates", "backward", "image_struct", "tile_configs", "lib",
 ]
 
 
@@ -257,6 +257,22 @@ def denoise(ks, xs, sigmas=20.0 / 255.0, gammas=2.0, scale_maps=False, flags=0, 
     if single:
         return (ys[0], maps[0]) if scale_maps else ys[0]
     return out
+
+
+def estimates(ks, xs, maps, flags=0, stream=None):
+    """Per-direction estimates at the scales `maps` (uint8 [D, H, W] device
+    tensors from a forward) selected, with the forward's walk for the same
+    flags: the values that forward aggregated.  xs: a device tensor or a list.
+    Returns float32 [D, H, W] device tensors (one, or a list)."""
+    torch = _torch()
+    single = not isinstance(xs, (list, tuple))
+    xs = [xs] if single else list(xs)
+    maps = [maps] if single else list(maps)
+    outs = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.float32, device=x.device) for x in xs]
+    arr = (LpaImage * len(xs))(*[image_struct(x, x, m, 1.0, 1.0) for x, m in zip(xs, maps)])
+    est_p = (ctypes.c_void_p * len(xs))(*[o.data_ptr() for o in outs])
+    check(lib().lpa_ici_estimates(ks.handle, arr, est_p, len(xs), int(flags), _stream_ptr(stream)))
+    return outs[0] if single else outs
 
 
 def backward(ks, xs, ys, maps, dys, sigmas=20.0 / 255.0, gammas=2.0, dkernel=None, stream=None):

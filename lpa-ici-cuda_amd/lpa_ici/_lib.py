@@ -76,6 +76,7 @@ SIGNATURES = {
     "lpa_tile_config_get": (_I, [_I, ctypes.POINTER(TileConfig)]),
     "lpa_kset_set_tile_config": (_I, [_P, _I]),
     "lpa_kset_get_tile_config": (_I, [_P]),
+    "lpa_ici_estimates": (_I, [_P, ctypes.POINTER(LpaImage), _P, _I, ctypes.c_uint, _P]),
     "lpa_ici_backward": (_I, [_P, ctypes.POINTER(LpaImage), _P, _P, _P, _I, _P]),
     "lpa_load_kernels": (_I, [ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                               _P, _P, _P, _P]),

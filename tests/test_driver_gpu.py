@@ -6,6 +6,7 @@ import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import GOLDEN, PKG, SYMMETRIC, kernel_path
@@ -29,3 +30,27 @@ def test_driver_barbara(tmp_path):
     assert "Calculated PSNR of img_denoised: 25.0736" in r.stdout
     for f in (out, tmp_path / "gray_image.hdr", tmp_path / "gray_image_noisy.hdr"):
         assert f.exists() and f.stat().st_size > 512 * 512 * 4
+
+
+def test_driver_debug_outputs(tmp_path):
+    """--debug writes the reference's per-direction debugging outputs: each
+    direction's selected estimate and its scale map s/(S-1)."""
+    from test_host import decode_hdr
+    dbg = tmp_path / "dbg"
+    dbg.mkdir()
+    out = tmp_path / "cameraman_denoised.hdr"
+    r = subprocess.run([os.path.join(PKG, "bin", "lpa_ici_denoise"), "-i",
+                        os.path.join(GOLDEN, "images", "cameraman.png"), "-k", kernel_path(SYMMETRIC), "-o", str(out),
+                        "--debug", str(dbg)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Saved 32 per-direction estimates and scale maps" in r.stdout
+    for d in range(16):
+        est = decode_hdr(dbg / f"gray_image_denoised_theta_{d}.hdr")[1]
+        smap = decode_hdr(dbg / f"scale_map_theta_{d}.hdr")[1]
+        assert est.shape == smap.shape == (256, 256)
+        # s/(S-1) for S = 7, within the RGBE mantissa's 1/256 of the exponent
+        levels = np.arange(7) / 6.0
+        assert np.all(np.abs(smap[..., None] - levels).min(axis=-1) <= 1.0 / 128)
+        assert 0.0 <= est.min() and est.max() <= 1.5
+    den = decode_hdr(out)[1]
+    assert abs(float(den.mean()) - float(decode_hdr(dbg / "gray_image_denoised_theta_0.hdr")[1].mean())) < 0.05

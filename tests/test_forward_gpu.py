@@ -270,3 +270,57 @@ def test_autotune_installs_fastest(ks, tmp_path):
         assert again == best and t2 == {}  # served from the cache
     finally:
         ks.set_tile_config(-1)
+
+
+def reaggregate(ks, est, maps, sigma=SIGMA):
+    """y from per-direction estimates in the forward's order and rounding:
+    num += fl(est * inv), den += inv over d, then num / den (float32)."""
+    var = ks.variances(sigma)
+    inv = (np.float32(1.0) / var).astype(np.float32)
+    num = np.zeros(est.shape[1:], np.float32)
+    den = np.zeros(est.shape[1:], np.float32)
+    for d in range(ks.D):
+        w = inv[d * ks.S + maps[d].astype(np.int64)]
+        num = (num + (est[d] * w).astype(np.float32)).astype(np.float32)
+        den = (den + w).astype(np.float32)
+    return (num / den).astype(np.float32)
+
+
+@pytest.mark.parametrize("name,flags", [(SYMMETRIC, 0), (SYMMETRIC, lpa_ici.FWD_RING_FLOAT),
+                                        (SYMMETRIC, lpa_ici.FWD_WEIGHTED), (SYMMETRIC, lpa_ici.FWD_EXACT),
+                                        (GAUSSIAN_M20, 0)])
+def test_estimates_are_the_aggregated_values(name, flags):
+    """The per-direction estimates at the selected scales, aggregated on the
+    host in the forward's order, give the forward's output bit for bit."""
+    ksx = kernels(name)
+    x = noisy_synth(150, 110, seed=9)
+    xd = dev(x)
+    y, m = lpa_ici.denoise(ksx, xd, SIGMA, GAMMA, scale_maps=True, flags=flags)
+    est = lpa_ici.estimates(ksx, xd, m, flags=flags)
+    torch.cuda.synchronize()
+    maps = m.cpu().numpy()
+    assert np.array_equal(reaggregate(ksx, est.cpu().numpy(), maps).view(np.uint32), y.cpu().numpy().view(np.uint32))
+
+
+def test_estimates_fixed_point_bitexact(ks):
+    """Default walk: equal to the oracle's fixed-point restatement, estimate by
+    estimate (the reference's per-direction debugging outputs)."""
+    F, rng = fixed_point(ks)
+    clean = lpa_ici.synth_image(300, 200, 2)
+    x = lpa_ici.make_noisy(clean, SIGMA, SEED)
+    xd = dev(x)
+    _, m = lpa_ici.denoise(ks, xd, SIGMA, GAMMA, scale_maps=True)
+    est = lpa_ici.estimates(ks, [xd], [m])[0].cpu().numpy()
+    _, mo, eo = oracle.walk_q(x, ks, ks.variances(SIGMA), GAMMA, F, rng, want_est=True)
+    assert np.array_equal(m.cpu().numpy(), mo)
+    assert np.array_equal(est.view(np.uint32), eo.view(np.uint32))
+
+
+def test_estimates_need_a_scale_map(ks):
+    xd = dev(noisy_synth(16, 16, seed=1))
+    import ctypes
+    arr = (lpa_ici.LpaImage * 1)(lpa_ici.image_struct(xd, xd, None, SIGMA, GAMMA))
+    out = torch.empty((ks.D, 16, 16), dtype=torch.float32, device="cuda")
+    est_p = (ctypes.c_void_p * 1)(out.data_ptr())
+    st = lpa_ici.lib().lpa_ici_estimates(ks.handle, arr, est_p, 1, 0, None)
+    assert st == 1 and b"scale map" in lpa_ici.lib().lpa_last_error()
