@@ -229,6 +229,18 @@ lpa_status lpa_load_image(const char* path, int* width, int* height, float* out,
  * 1e-32 (negatives included) are written as 0. */
 lpa_status lpa_write_hdr(const char* path, int width, int height, const float* data);
 
+/* Multichannel input and output (the reference loads with one channel only,
+ * src/v0_0_lpa_ici_2D_naive.cpp:254-256).  lpa_load_image_channels:
+ * stbi_loadf(path, &w, &h, &c, channels) semantics for channels 1 or 3
+ * (interleaved; gray PNGs replicate to RGB, palettes expand, alpha is
+ * dropped, each channel goes through powf(v/255, 2.2)).
+ * lpa_write_hdr_channels: RGBE with a shared exponent per pixel, as
+ * stbi_write_hdr (stb_image_write.h:639-654); a pixel whose largest component
+ * is below 1e-32 is written as 0. */
+lpa_status lpa_load_image_channels(const char* path, int channels, int* width, int* height, float* out,
+                                   long long capacity);
+lpa_status lpa_write_hdr_channels(const char* path, int width, int height, int channels, const float* data);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,15 +18,16 @@ int main(int argc, char **argv)
 {
     const Config cfg = parse_args(argc, argv, "lpa_ici_denoise_gpu");
 
-    int width = 0, height = 0, channels = 1;
+    int width = 0, height = 0;
+    const int channels = cfg.channels;
     std::vector<float> img_gray;
-    if (lpa_load_image(cfg.image_file.c_str(), &width, &height, nullptr, 0) != LPA_OK) {
+    if (lpa_load_image_channels(cfg.image_file.c_str(), channels, &width, &height, nullptr, 0) != LPA_OK) {
         std::cerr << "Error in loading the image " << cfg.image_file << "\n";
         return 1;
     }
-    img_gray.resize((size_t)width * height);
-    if (lpa_load_image(cfg.image_file.c_str(), &width, &height, img_gray.data(), (long long)img_gray.size()) !=
-        LPA_OK) {
+    img_gray.resize((size_t)width * height * channels);
+    if (lpa_load_image_channels(cfg.image_file.c_str(), channels, &width, &height, img_gray.data(),
+                                (long long)img_gray.size()) != LPA_OK) {
         std::cerr << "Error in loading the image " << cfg.image_file << "\n";
         return 1;
     }
@@ -35,14 +36,14 @@ int main(int argc, char **argv)
 
     const std::string gray_file = sibling_path(cfg.output_file, "gray_image.hdr");
     std::cout << "Saving gray image as " << gray_file << "\n";
-    lpa_write_hdr(gray_file.c_str(), width, height, img_gray.data());
+    lpa_write_hdr_channels(gray_file.c_str(), width, height, channels, img_gray.data());
 
     const int img_size = width * height * channels;
     std::vector<float> noisy(img_gray);
     create_noisy_image(img_gray.data(), noisy.data(), img_size, cfg.sigma_noise, cfg.noise_seed);
     const std::string noisy_file = sibling_path(cfg.output_file, "gray_image_noisy.hdr");
     std::cout << "Saving noisy gray image as " << noisy_file << "\n";
-    lpa_write_hdr(noisy_file.c_str(), width, height, noisy.data());
+    lpa_write_hdr_channels(noisy_file.c_str(), width, height, channels, noisy.data());
 
     const double psnr_noisy = calculate_PSNR(img_gray.data(), noisy.data(), img_size);
     std::cout << "Calculated PSNR of host_img_gray_noisy: " << psnr_noisy << "\n";
@@ -57,19 +58,26 @@ int main(int argc, char **argv)
 
     std::vector<float> denoised(img_size, 0.0f), wbuf(img_size, 0.0f);
     Timer t;
-    anisotropic_lpa_ici_gpu_fused(noisy.data(), denoised.data(), width, height, channels, wbuf.data(), num_dirs,
-                                  num_scales, weights.data(), (int)weights.size(), offsets.data(), rows.data(),
-                                  cols.data(), variances.data(), cfg.ici_gamma);
+    if (channels == 1)
+        anisotropic_lpa_ici_gpu_fused(noisy.data(), denoised.data(), width, height, channels, wbuf.data(), num_dirs,
+                                      num_scales, weights.data(), (int)weights.size(), offsets.data(), rows.data(),
+                                      cols.data(), variances.data(), cfg.ici_gamma);
+    else
+        lpa_ici_denoise_channels(noisy.data(), denoised.data(), width, height, channels, num_dirs, num_scales,
+                                 weights.data(), offsets.data(), rows.data(), cols.data(), variances.data(),
+                                 cfg.ici_gamma);
     const double seconds = t.elapsed();
     std::cout << "Time taken for lpa function execution: " << seconds << " seconds\n";
 
     const double psnr_denoised = calculate_PSNR(img_gray.data(), denoised.data(), img_size);
     std::cout << "Calculated PSNR of img_denoised: " << psnr_denoised << "\n";
     std::cout << "Saving denoised image as " << cfg.output_file << "\n";
-    if (lpa_write_hdr(cfg.output_file.c_str(), width, height, denoised.data()) != LPA_OK)
+    if (lpa_write_hdr_channels(cfg.output_file.c_str(), width, height, channels, denoised.data()) != LPA_OK)
         std::cerr << "Error: " << lpa_last_error() << "\n";
 
-    if (!cfg.debug_dir.empty())
+    if (!cfg.debug_dir.empty() && channels != 1)
+        std::cerr << "--debug applies to one channel; skipped\n";
+    else if (!cfg.debug_dir.empty())
         write_direction_outputs(noisy.data(), width, height, num_dirs, num_scales, weights.data(), offsets.data(),
                                 rows.data(), cols.data(), variances.data(), cfg.ici_gamma, cfg.debug_dir);
 

@@ -15,6 +15,7 @@ struct Config {
     unsigned int noise_seed = 20250910u;
     float ici_gamma = 2.0f;
     std::string debug_dir;  // non-empty: write the per-direction debugging outputs there
+    int channels = 1;       // 1: gray (the reference's only mode); 3: RGB, each channel denoised
 };
 
 // Directory part of a path, trailing slash included ("" if none).
@@ -38,6 +39,7 @@ inline void print_usage(const char *prog)
               << "  -g, --gamma <value>    ICI threshold (default " << d.ici_gamma << ")\n"
               << "      --seed <value>     noise seed (default " << d.noise_seed << ")\n"
               << "      --debug <dir>      also write each direction's estimate and scale map as .hdr in <dir>\n"
+              << "  -c, --channels <1|3>   1: gray (default); 3: RGB, every channel denoised on its own\n"
               << "  -h, --help             show this message\n";
 }
 
@@ -82,6 +84,11 @@ inline Config parse_args(int argc, char **argv, const std::string &version)
             cfg.noise_seed = static_cast<unsigned int>(v);
         } else if (opt == "--debug")
             cfg.debug_dir = val;
+        else if (opt == "-c" || opt == "--channels") {
+            const std::string v = val;
+            if (v != "1" && v != "3") die("--channels needs 1 or 3, got '" + v + "'");
+            cfg.channels = v == "3" ? 3 : 1;
+        }
         else
             die("unknown option '" + opt + "'");
     }

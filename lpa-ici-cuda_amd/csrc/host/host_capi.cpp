@@ -101,6 +101,35 @@ lpa_status lpa_load_image(const char *path, int *width, int *height, float *out,
     return LPA_OK;
 }
 
+lpa_status lpa_load_image_channels(const char *path, int channels, int *width, int *height, float *out,
+                                   long long capacity)
+{
+    if (!path || !width || !height) return lpa::fail(LPA_EINVAL, "null argument");
+    if (channels != 1 && channels != 3) return lpa::fail(LPA_EINVAL, "channels must be 1 or 3");
+    std::vector<float> img;
+    int w = 0, h = 0;
+    std::string err = lpa::load_png(path, channels, w, h, img);
+    if (!err.empty()) return lpa::fail(err == "can't fopen" ? LPA_EIO : LPA_EINVAL, err + ": " + path);
+    *width = w;
+    *height = h;
+    if (out) {
+        if ((long long)img.size() > capacity) return lpa::fail(LPA_EINVAL, "output buffer too small");
+        std::memcpy(out, img.data(), img.size() * sizeof(float));
+    }
+    lpa::clear_error();
+    return LPA_OK;
+}
+
+lpa_status lpa_write_hdr_channels(const char *path, int width, int height, int channels, const float *data)
+{
+    if (!path || !data || width <= 0 || height <= 0 || (channels != 1 && channels != 3))
+        return lpa::fail(LPA_EINVAL, "bad argument");
+    if (!lpa::write_hdr(path, width, height, channels, data))
+        return lpa::fail(LPA_EIO, std::string("cannot write ") + path);
+    lpa::clear_error();
+    return LPA_OK;
+}
+
 lpa_status lpa_write_hdr(const char *path, int width, int height, const float *data)
 {
     if (!path || !data || width <= 0 || height <= 0) return lpa::fail(LPA_EINVAL, "bad argument");

@@ -110,11 +110,15 @@ static int paeth(int a, int b, int c)
     return pb <= pc ? b : c;
 }
 
-// Gray value stb_image produces when asked for one channel: RGB folds to
-// (77R + 150G + 29B) >> 8 on 8-bit values (stb_image.h:1746-1749), 16-bit
-// samples keep their high byte, alpha is dropped.
-std::string load_png_gray(const std::string &path, int &width, int &height, std::vector<float> &out)
+// 8-bit PNG to float with stbi_loadf(path, &w, &h, &c, channels) semantics
+// for channels 1 or 3 (interleaved).  One channel: RGB folds to
+// (77R + 150G + 29B) >> 8 (stb_image.h:1746-1749); three: gray replicates,
+// palette entries expand.  16-bit samples keep their high byte, alpha is
+// dropped, and every channel goes through powf(v / 255, 2.2)
+// (stbi__ldr_to_hdr, stb_image.h:1858-1880).
+std::string load_png(const std::string &path, int channels, int &width, int &height, std::vector<float> &out)
 {
+    if (channels != 1 && channels != 3) return "only 1 or 3 channels";
     std::ifstream f(path, std::ios::binary);
     if (!f.is_open()) return "can't fopen";
     std::vector<unsigned char> b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
@@ -174,28 +178,47 @@ std::string load_png_gray(const std::string &path, int &width, int &height, std:
     }
     width = (int)W;
     height = (int)H;
-    out.resize((size_t)W * H);
+    out.resize((size_t)W * H * channels);
+    auto to_float = [](int v) { return std::pow((float)v / 255.0f, 2.2f) * 1.0f; };  // pow in float, scale 1
     for (size_t i = 0; i < (size_t)W * H; ++i) {
         const unsigned char *p = &img[i * bpp];
         auto sample = [&](int k) { return (int)p[k * bytes]; };  // high byte of 16-bit samples
-        int g;
+        int rgb[3];
         if (ctype == 3) {
             size_t idx = p[0];
             if (3 * idx + 2 >= plte.size()) return "bad PNG palette index";
-            g = (plte[3 * idx] * 77 + plte[3 * idx + 1] * 150 + plte[3 * idx + 2] * 29) >> 8;
+            rgb[0] = plte[3 * idx];
+            rgb[1] = plte[3 * idx + 1];
+            rgb[2] = plte[3 * idx + 2];
         } else if (ch >= 3) {
-            g = (sample(0) * 77 + sample(1) * 150 + sample(2) * 29) >> 8;
+            rgb[0] = sample(0);
+            rgb[1] = sample(1);
+            rgb[2] = sample(2);
         } else {
-            g = sample(0);
+            rgb[0] = rgb[1] = rgb[2] = sample(0);
         }
-        // stbi__ldr_to_hdr (stb_image.h:1858-1880): pow in float, scale 1.
-        out[i] = std::pow((float)g / 255.0f, 2.2f) * 1.0f;
+        if (channels == 1) {
+            const int g = (ctype == 3 || ch >= 3) ? (rgb[0] * 77 + rgb[1] * 150 + rgb[2] * 29) >> 8 : rgb[0];
+            out[i] = to_float(g);
+        } else {
+            for (int c = 0; c < 3; ++c) out[3 * i + c] = to_float(rgb[c]);
+        }
     }
     return "";
 }
 
-bool write_hdr(const std::string &path, int width, int height, const float *data)
+std::string load_png_gray(const std::string &path, int &width, int &height, std::vector<float> &out)
 {
+    return load_png(path, 1, width, height, out);
+}
+
+// Radiance RGBE with a shared exponent per pixel (stb_image_write.h:639-654):
+// the largest component m sets the exponent, every component is written as
+// (unsigned char)(v * frexp(m) * 256 / m); m below 1e-32 (negatives
+// included) writes 0.  One channel is replicated to R, G and B.
+bool write_hdr(const std::string &path, int width, int height, int channels, const float *data)
+{
+    if (channels != 1 && channels != 3) return false;
     FILE *f = std::fopen(path.c_str(), "wb");
     if (!f) return false;
     std::fprintf(f, "#?RADIANCE\n# Written by lpa_ici\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", height, width);
@@ -203,21 +226,29 @@ bool write_hdr(const std::string &path, int width, int height, const float *data
     bool ok = true;
     for (int y = 0; y < height && ok; ++y) {
         for (int x = 0; x < width; ++x) {
-            float v = data[(size_t)y * width + x];
+            const float *v = &data[((size_t)y * width + x) * channels];
+            const float r = v[0], g = channels == 3 ? v[1] : v[0], b = channels == 3 ? v[2] : v[0];
+            const float m = std::max(r, std::max(g, b));
             unsigned char *q = &line[(size_t)x * 4];
-            if (!(v >= 1e-32f)) {
+            if (!(m >= 1e-32f)) {
                 q[0] = q[1] = q[2] = q[3] = 0;
             } else {
                 int e;
-                float m = std::frexp(v, &e) * 256.0f / v;
-                unsigned char c = (unsigned char)(v * m);
-                q[0] = q[1] = q[2] = c;
+                const float n = std::frexp(m, &e) * 256.0f / m;
+                q[0] = (unsigned char)(r * n);
+                q[1] = (unsigned char)(g * n);
+                q[2] = (unsigned char)(b * n);
                 q[3] = (unsigned char)(e + 128);
             }
         }
         ok = std::fwrite(line.data(), 1, line.size(), f) == line.size();
     }
     return std::fclose(f) == 0 && ok;
+}
+
+bool write_hdr(const std::string &path, int width, int height, const float *data)
+{
+    return write_hdr(path, width, height, 1, data);
 }
 
 // ---------------------------------------------------------------------------

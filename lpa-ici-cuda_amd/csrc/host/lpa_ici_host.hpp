@@ -48,6 +48,16 @@ void write_direction_outputs(const float *host_img_noisy, int width, int height,
                              const int *host_cols, const float *host_kernel_variances, float ici_gamma,
                              const std::string &dir);
 
+// Multichannel form of the orchestrator (the reference accepts one channel
+// only): the interleaved channels are split into planes and denoised
+// together in one persistent launch over the C ABI, each with the same
+// kernel set, variances and gamma.  Prints the reference's timing line for
+// the same window.  Exits with EXIT_FAILURE on any failure.
+void lpa_ici_denoise_channels(const float *host_img_noisy, float *host_img_denoised, int width, int height,
+                              int channels, int num_dirs, int num_scales, const float *host_kernel_weights,
+                              const int *host_offsets, const int *host_rows, const int *host_cols,
+                              const float *host_kernel_variances, float ici_gamma);
+
 namespace lpa {
 
 // Read a whole text file, transparently gunzipping names ending in ".gz".
@@ -60,8 +70,13 @@ std::string load_kernels(const std::string &path, std::vector<float> &w, std::ve
 // 8-bit PNG to gray float (stbi_loadf(.., 1) semantics); "" or an error.
 std::string load_png_gray(const std::string &path, int &width, int &height, std::vector<float> &out);
 
-// Radiance RGBE writer, negatives and values < 1e-32 written as 0.
+// 8-bit PNG to float with stbi_loadf(.., channels) semantics (1 or 3).
+std::string load_png(const std::string &path, int channels, int &width, int &height, std::vector<float> &out);
+
+// Radiance RGBE writer (shared exponent), 1 or 3 interleaved channels;
+// pixels whose largest component is below 1e-32 (negatives included) are 0.
 bool write_hdr(const std::string &path, int width, int height, const float *data);
+bool write_hdr(const std::string &path, int width, int height, int channels, const float *data);
 
 // Deterministic synthetic image in [0,1].
 void synth_image(float *out, int width, int height, unsigned seed);

@@ -177,17 +177,28 @@ def synth_image(width, height, seed):
     return out
 
 
-def load_image(path):
+def load_image(path, channels=1):
+    """stbi_loadf semantics: [H, W] for one channel, [H, W, 3] for three."""
     W, H = ctypes.c_int(), ctypes.c_int()
-    check(lib().lpa_load_image(path.encode(), ctypes.byref(W), ctypes.byref(H), None, 0))
-    out = np.empty((H.value, W.value), np.float32)
-    check(lib().lpa_load_image(path.encode(), ctypes.byref(W), ctypes.byref(H), _ptr(out), out.size))
+    if channels == 1:
+        check(lib().lpa_load_image(path.encode(), ctypes.byref(W), ctypes.byref(H), None, 0))
+        out = np.empty((H.value, W.value), np.float32)
+        check(lib().lpa_load_image(path.encode(), ctypes.byref(W), ctypes.byref(H), _ptr(out), out.size))
+        return out
+    check(lib().lpa_load_image_channels(path.encode(), channels, ctypes.byref(W), ctypes.byref(H), None, 0))
+    out = np.empty((H.value, W.value, channels), np.float32)
+    check(lib().lpa_load_image_channels(path.encode(), channels, ctypes.byref(W), ctypes.byref(H), _ptr(out),
+                                        out.size))
     return out
 
 
 def write_hdr(path, img):
+    """[H, W] (replicated to RGB) or [H, W, 3]."""
     img = _f32(img)
-    check(lib().lpa_write_hdr(path.encode(), img.shape[1], img.shape[0], _ptr(img)))
+    if img.ndim == 2:
+        check(lib().lpa_write_hdr(path.encode(), img.shape[1], img.shape[0], _ptr(img)))
+    else:
+        check(lib().lpa_write_hdr_channels(path.encode(), img.shape[1], img.shape[0], img.shape[2], _ptr(img)))
 
 
 def tile_configs():

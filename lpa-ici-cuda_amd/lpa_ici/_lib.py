@@ -86,6 +86,8 @@ SIGNATURES = {
     "lpa_synth_image": (None, [_P, _I, _I, ctypes.c_uint]),
     "lpa_load_image": (_I, [ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I), _P, _LL]),
     "lpa_write_hdr": (_I, [ctypes.c_char_p, _I, _I, _P]),
+    "lpa_load_image_channels": (_I, [ctypes.c_char_p, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _P, _LL]),
+    "lpa_write_hdr_channels": (_I, [ctypes.c_char_p, _I, _I, _I, _P]),
 }
 
 _lib = None

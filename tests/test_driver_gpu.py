@@ -54,3 +54,24 @@ def test_driver_debug_outputs(tmp_path):
         assert 0.0 <= est.min() and est.max() <= 1.5
     den = decode_hdr(out)[1]
     assert abs(float(den.mean()) - float(decode_hdr(dbg / "gray_image_denoised_theta_0.hdr")[1].mean())) < 0.05
+
+
+def test_driver_rgb(tmp_path):
+    """--channels 3: RGB in and out; each channel denoised in one batched
+    launch; the PSNR over all samples improves as for gray."""
+    from test_host import decode_hdr, write_png
+    import lpa_ici
+    rgb = np.stack([(lpa_ici.synth_image(200, 150, s) * 255).astype(np.uint8) for s in (1, 2, 3)], axis=-1)
+    src = tmp_path / "rgb.png"
+    write_png(src, rgb)
+    out = tmp_path / "rgb_denoised.hdr"
+    r = subprocess.run([os.path.join(PKG, "bin", "lpa_ici_denoise"), "-i", str(src), "-k", kernel_path(SYMMETRIC),
+                        "-o", str(out), "--channels", "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Loaded image with a width of 200 px, a height of 150 px and (imported) 3 channels" in r.stdout
+    info = json.loads(r.stdout.splitlines()[-1])
+    assert info["psnr_denoised"] > info["psnr_noisy"] + 4
+    head, _ = decode_hdr(out)
+    data = open(out, "rb").read()
+    q = np.frombuffer(data.partition(b"\n\n")[2].partition(b"\n")[2], np.uint8).reshape(150, 200, 4)
+    assert not np.array_equal(q[..., 0], q[..., 1])  # the channels differ

@@ -319,3 +319,64 @@ def test_driver_help_and_errors(tmp_path):
     assert r.returncode == 1 and "Failed to open" in r.stderr
     assert (tmp_path / "gray_image.hdr").exists() and (tmp_path / "gray_image_noisy.hdr").exists()
     assert "Calculated PSNR of host_img_gray_noisy: 22.1245" in r.stdout
+
+
+def write_png(path, img):
+    """8-bit PNG (gray [H, W] or RGB [H, W, 3]), filter 0, via zlib."""
+    import struct
+    import zlib
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    ctype = 0 if img.ndim == 2 else 2
+    raw = b"".join(b"\x00" + img[y].tobytes() for y in range(h))
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xffffffff)
+    data = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, ctype, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
+    open(path, "wb").write(data)
+
+
+def test_load_image_channels(tmp_path):
+    """stbi_loadf semantics for 1 and 3 channels: a gray PNG loaded as RGB is
+    its gray load replicated; an RGB PNG loaded as gray is the gray load of
+    the PNG holding (77R + 150G + 29B) >> 8; both bit for bit."""
+    rng = np.random.default_rng(3)
+    rgb = rng.integers(0, 256, size=(7, 11, 3), dtype=np.uint8)
+    gray = rng.integers(0, 256, size=(7, 11), dtype=np.uint8)
+    write_png(tmp_path / "rgb.png", rgb)
+    write_png(tmp_path / "gray.png", gray)
+    folded = ((rgb[..., 0].astype(np.int32) * 77 + rgb[..., 1].astype(np.int32) * 150 +
+               rgb[..., 2].astype(np.int32) * 29) >> 8).astype(np.uint8)
+    write_png(tmp_path / "folded.png", folded)
+    g1 = lpa_ici.load_image(str(tmp_path / "gray.png"))
+    g3 = lpa_ici.load_image(str(tmp_path / "gray.png"), channels=3)
+    assert g3.shape == (7, 11, 3) and all(np.array_equal(g3[..., c], g1) for c in range(3))
+    r1 = lpa_ici.load_image(str(tmp_path / "rgb.png"))
+    assert np.array_equal(r1, lpa_ici.load_image(str(tmp_path / "folded.png")))
+    r3 = lpa_ici.load_image(str(tmp_path / "rgb.png"), channels=3)
+    # each RGB channel is the gray curve at that channel's 8-bit value
+    for c in range(3):
+        write_png(tmp_path / f"c{c}.png", rgb[..., c])
+        assert np.array_equal(r3[..., c], lpa_ici.load_image(str(tmp_path / f"c{c}.png")))
+    with pytest.raises(lpa_ici.LpaError):
+        lpa_ici.load_image(str(tmp_path / "rgb.png"), channels=2)
+
+
+def test_hdr_writer_rgb(tmp_path):
+    """Shared exponent per pixel: every component within 1/128 of the largest."""
+    rng = np.random.default_rng(4)
+    img = rng.random((5, 8, 3)).astype(np.float32) * 3
+    img[0, 0] = [-1.0, 0.0, 0.0]
+    p = str(tmp_path / "c.hdr")
+    lpa_ici.write_hdr(p, img)
+    data = open(p, "rb").read()
+    _, _, body = data.partition(b"\n\n")
+    _, _, pix = body.partition(b"\n")
+    q = np.frombuffer(pix, np.uint8).reshape(5, 8, 4).astype(np.float64)
+    scale = np.where(q[..., 3] > 0, np.ldexp(1.0, (q[..., 3] - 136).astype(int)), 0.0)
+    val = (q[..., :3] + 0.5) * scale[..., None] * (q[..., :3] > 0)
+    m = img.max(axis=2, keepdims=True)
+    ok = m[..., 0] > 0
+    assert np.all(np.abs(val - np.maximum(img, 0))[ok] <= (m / 128)[ok] + 1e-9)
+    assert np.all(q[0, 0] == 0)
