@@ -399,8 +399,16 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings,
                     sv[i][j] = in ? as_global(img.map)[(size_t)d * plane + (size_t)gy * img.W + gx] : 255;
                 }
             if constexpr (RING) {
-                for (int s = S - 1; s >= 0; --s) {
-                    bool any = false;
+                // The deepest scale selected in the window: the walk starts
+                // there, and every map from it down is non-empty.
+                int deepest = -1;
+#pragma unroll
+                for (int i = 0; i < kSlotRows; ++i)
+#pragma unroll
+                    for (int j = 0; j < kSlotCols; ++j)
+                        if (sv[i][j] < S && qv[i][j] != 0.0f) deepest = max(deepest, sv[i][j]);
+                const int top = block_max<DXT>(deepest);
+                for (int s = top; s >= 0; --s) {
                     __syncthreads();  // previous ring sums done with the map
                     const int k = d * S + s;
                     const float cw = as_const(inv_t)[k] * as_const(rings.scale)[k];
@@ -409,13 +417,12 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings,
 #pragma unroll
                         for (int j = 0; j < kSlotCols; ++j) {
                             const int r = r0 + 16 * i, c = c0 + 64 * j;
-                            if (r < fh && c < fw) {
-                                // the first (largest) scale writes every element
-                                if (s == S - 1 || sv[i][j] == s) mt[r * lp + c] = sv[i][j] == s ? qv[i][j] * cw : 0.0f;
-                                any |= sv[i][j] >= s && sv[i][j] < S && qv[i][j] != 0.0f;
-                            }
+                            // the first step writes every element, later ones
+                            // only those that join the map
+                            if (r < fh && c < fw && (s == top || sv[i][j] == s))
+                                mt[r * lp + c] = sv[i][j] == s ? qv[i][j] * cw : 0.0f;
                         }
-                    if (!block_vote<DXT, false>(any)) continue;
+                    __syncthreads();
                     correlate_runs<DXP, false, 1, false>(lane_m, lp, rings.flip, nullptr, load_const(rings.kern + k), dx);
                 }
             } else {

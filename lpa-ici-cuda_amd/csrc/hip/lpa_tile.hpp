@@ -284,6 +284,23 @@ __device__ __forceinline__ bool block_vote(bool v)
 }
 __device__ __forceinline__ bool block_all(bool v) { return block_vote<kThreads, true>(v); }
 
+// Work-group maximum of v, wave-uniform in every lane.  Two barriers: one
+// before the slots are read, one after, so back-to-back calls do not race.
+template <int NT>
+__device__ __forceinline__ int block_max(int v)
+{
+    __shared__ int wave_max[NT / 64];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int r = wave_max[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) r = max(r, wave_max[i]);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(r);
+}
+
 // Stage the (TW + 2R) x (TH + 2R) window (as stage_tile) as row-prefix sums
 // of fixed-point values: q[r * lp + j] = Q[r][j] for j = 0 .. TW + 2R
 // (Q[r][0] = 0).  Each wave takes whole rows, two values per lane, and scans
