@@ -303,8 +303,8 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the denoiser has no multi-GPU path in this build "
-              "(images shard across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the denoiser has no multi-GPU path "
+              "(images shard across GPUs with no communication)", file=sys.stderr)
         sys.exit(2)
     torch.cuda.init()
     stream = torch.cuda.Stream()

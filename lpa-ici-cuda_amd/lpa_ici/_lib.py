@@ -98,7 +98,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C {PKG_DIR}` (or __graft_entry__.build())")
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C {PKG_DIR}`")
         handle = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)
