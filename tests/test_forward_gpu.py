@@ -324,3 +324,25 @@ def test_estimates_need_a_scale_map(ks):
     est_p = (ctypes.c_void_p * 1)(out.data_ptr())
     st = lpa_ici.lib().lpa_ici_estimates(ks.handle, arr, est_p, 1, 0, None)
     assert st == 1 and b"scale map" in lpa_ici.lib().lpa_last_error()
+
+
+def test_full_size_windows_bitexact(ks):
+    """Config 2's size (one 4096x4096 image): the walk is local (a pixel reads
+    at most 31 px away), so the oracle's fixed-point restatement on a crop
+    with a 31-px margin must give the GPU's output in the crop's interior bit
+    for bit.  Three windows: a corner, an edge and the interior."""
+    F, rng = fixed_point(ks)
+    clean = lpa_ici.synth_image(4096, 4096, 7)
+    x = lpa_ici.make_noisy(clean, SIGMA, SEED)
+    y, maps = run(ks, x)
+    var = ks.variances(SIGMA)
+    R = 31
+    for (r0, c0, n) in [(0, 0, 96), (0, 2000, 96), (3000, 1234, 160)]:
+        ra, ca = max(0, r0 - R), max(0, c0 - R)
+        rb, cb = min(4096, r0 + n + R), min(4096, c0 + n + R)
+        yo, mo = oracle.walk_q(np.ascontiguousarray(x[ra:rb, ca:cb]), ks, var, GAMMA, F, rng)
+        inner = (slice(r0 - ra, r0 - ra + n), slice(c0 - ca, c0 - ca + n))
+        # a crop edge that is not the image edge sees zeros beyond it: only
+        # pixels at least R inside the crop's own edges are compared
+        assert np.array_equal(y[r0:r0 + n, c0:c0 + n].view(np.uint32), yo[inner].view(np.uint32)), (r0, c0)
+        assert np.array_equal(maps[:, r0:r0 + n, c0:c0 + n], mo[(slice(None),) + inner]), (r0, c0)
