@@ -34,11 +34,13 @@ struct lpa_kset {
     // Adjoint tables (see lpa::DevAdjoint).
     std::vector<int4> runs_flip;
     std::vector<float> weights_flip;
-    std::vector<int2> tap_pos, kern_w;
+    std::vector<int> tap_lds;
+    std::vector<int4> kern_w;
     std::vector<int> tap_kernel;
     int4 *d_runs_flip = nullptr;
     float *d_weights_flip = nullptr;
-    int2 *d_tap_pos = nullptr, *d_kern_w = nullptr;
+    int *d_tap_lds = nullptr;
+    int4 *d_kern_w = nullptr;
     int *d_tap_kernel = nullptr;
     float *d_sumsq = nullptr;
     // Ring form (see lpa::DevRings), when every kernel has uniform weights
@@ -71,7 +73,7 @@ struct lpa_kset {
     lpa::DevAdjoint adjoint() const
     {
         return lpa::DevAdjoint{lpa::DevKset{d_runs_flip, d_weights_flip, d_kern, D, S},
-                               d_tap_pos,
+                               d_tap_lds,
                                d_tap_kernel,
                                d_kern_w,
                                d_sumsq,
@@ -134,7 +136,10 @@ void pack_runs(lpa_kset *ks, const float *dense, const int *offsets, const int *
         }
         ks->kern.push_back(int2{first_run, (int)ks->runs.size() - first_run});
         const int first_w = first_run < (int)ks->runs.size() ? ks->runs[first_run].w : (int)ks->weights.size();
-        ks->kern_w.push_back(int2{first_w, (int)ks->weights.size() - first_w});
+        int pieces = 0;
+        for (int r = first_run; r < (int)ks->runs.size(); ++r)
+            pieces += (ks->runs[r].z + lpa::kGradPiece - 1) / lpa::kGradPiece;
+        ks->kern_w.push_back(int4{first_w, (int)ks->weights.size() - first_w, pieces, 0});
         ks->nnz.push_back(nz);
         ks->sumsq.push_back(ss);
         ks->num_nonzero += nz;
@@ -142,7 +147,8 @@ void pack_runs(lpa_kset *ks, const float *dense, const int *offsets, const int *
     ks->radius = radius;
     // Adjoint tables: tap positions and owners, and the rotated runs.
     ks->weights_flip.resize(ks->weights.size());
-    ks->tap_pos.resize(ks->weights.size());
+    ks->tap_lds.resize(ks->weights.size());
+    const int grad_pitch = lpa::lds_pitch(lpa::kBwdTileW, radius);
     ks->tap_kernel.resize(ks->weights.size());
     for (int k = 0; k < ks->K; ++k) {
         const int2 kr = ks->kern[k];
@@ -151,7 +157,7 @@ void pack_runs(lpa_kset *ks, const float *dense, const int *offsets, const int *
             ks->runs_flip.push_back(int4{-run.x, -(run.y + run.z - 1), run.z, run.w});
             for (int j = 0; j < run.z; ++j) {
                 ks->weights_flip[run.w + j] = ks->weights[run.w + run.z - 1 - j];
-                ks->tap_pos[run.w + j] = int2{run.x, run.y + j};
+                ks->tap_lds[run.w + j] = run.x * grad_pitch + run.y + j;
                 ks->tap_kernel[run.w + j] = k;
             }
         }
@@ -298,9 +304,9 @@ lpa_status ensure_uploaded(lpa_kset *ks)
     if (e == hipSuccess) e = hipMemcpy(ks->d_kern, ks->kern.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&ks->d_runs_flip, sizeof(int4) * nr);
     if (e == hipSuccess) e = hipMalloc(&ks->d_weights_flip, sizeof(float) * (nw + 4));
-    if (e == hipSuccess) e = hipMalloc(&ks->d_tap_pos, sizeof(int2) * nw);
+    if (e == hipSuccess) e = hipMalloc(&ks->d_tap_lds, sizeof(int) * nw);
     if (e == hipSuccess) e = hipMalloc(&ks->d_tap_kernel, sizeof(int) * nw);
-    if (e == hipSuccess) e = hipMalloc(&ks->d_kern_w, sizeof(int2) * K);
+    if (e == hipSuccess) e = hipMalloc(&ks->d_kern_w, sizeof(int4) * K);
     if (e == hipSuccess) e = hipMalloc(&ks->d_sumsq, sizeof(float) * K);
     if (e == hipSuccess) e = hipMemset(ks->d_weights_flip, 0, sizeof(float) * (nw + 4));
     if (e == hipSuccess && !ks->runs.empty())
@@ -308,11 +314,11 @@ lpa_status ensure_uploaded(lpa_kset *ks)
     if (e == hipSuccess && !ks->weights.empty()) {
         const size_t n = ks->weights.size();
         e = hipMemcpy(ks->d_weights_flip, ks->weights_flip.data(), sizeof(float) * n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(ks->d_tap_pos, ks->tap_pos.data(), sizeof(int2) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(ks->d_tap_lds, ks->tap_lds.data(), sizeof(int) * n, hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(ks->d_tap_kernel, ks->tap_kernel.data(), sizeof(int) * n, hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = hipMemcpy(ks->d_kern_w, ks->kern_w.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ks->d_kern_w, ks->kern_w.data(), sizeof(int4) * K, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ks->d_sumsq, ks->sumsq.data(), sizeof(float) * K, hipMemcpyHostToDevice);
     if (e == hipSuccess && ks->has_rings) {
         const size_t nrr = std::max<size_t>(1, ks->ring_runs.size());
@@ -478,7 +484,7 @@ void lpa_kset_destroy(lpa_kset_t ks)
     if (ks->d_runs) (void)hipFree(ks->d_runs);
     if (ks->d_weights) (void)hipFree(ks->d_weights);
     if (ks->d_kern) (void)hipFree(ks->d_kern);
-    for (void *p : {(void *)ks->d_runs_flip, (void *)ks->d_weights_flip, (void *)ks->d_tap_pos,
+    for (void *p : {(void *)ks->d_runs_flip, (void *)ks->d_weights_flip, (void *)ks->d_tap_lds,
                     (void *)ks->d_tap_kernel, (void *)ks->d_kern_w, (void *)ks->d_sumsq, (void *)ks->d_ring_runs,
                     (void *)ks->d_ring_runs_flip, (void *)ks->d_ring_kern, (void *)ks->d_ring_scale})
         if (p) (void)hipFree(p);

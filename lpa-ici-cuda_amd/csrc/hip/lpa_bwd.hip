@@ -12,10 +12,10 @@
 // Nothing of the forward but y and the uint8 scale maps is kept: yhat is
 // recomputed from the maps here.
 //
-// Pass 1 (one tile at a time): q, the recomputed yhat of the selected scale,
-// E and A, all from one walk over each selected kernel's runs per wave; the
-// wave's A partials are summed across its lanes by a reduce-scatter and
-// added into the wave's own row of `partial`.
+// Pass 1 (one tile at a time): q, the recomputed yhat of the selected scale
+// and E, then A per wave and selected kernel (a walk over the kernel's runs
+// for the whole wave, or the kernel's taps on the lanes for a few pixels),
+// added into the work-group's row of `partial`.
 // Pass 2 (dx): per direction and scale, the masked map q*inv*[s_d = s] over
 // tile + halo is correlated with the 180-degree rotated kernel (for ring-form
 // sets, nested masked maps with the rotated rings).
@@ -33,11 +33,11 @@ constexpr int kMaxScales = 16;
 // Pass-1 tile: 64 x 32 output pixels, 8 adjacent pixels per lane, 256 lanes.
 constexpr int GW = 64, GH = 32, GP = 8, GLX = GW / GP;
 static_assert(GLX * GH == kThreads, "pass-1 tile must use 256 lanes");
-// Run pieces the pass-1 bodies are unrolled for (longer runs are walked in
-// pieces of this length).
-constexpr int kGradPiece = 8;
 
-__host__ __device__ inline int grad_tab_offset(int R) { return lds_floats(GW, GH, R); }
+// The float tile (pitch lds_pitch(GW, R)) and the prefix table (one more
+// column, pitch lds_pitch(GW + 1, R)) share the start of the LDS; the [3][K]
+// table follows the larger of the two.
+__host__ __device__ inline int grad_tab_offset(int R) { return lds_floats(GW + 1, GH, R); }
 
 __global__ __launch_bounds__(kThreads) void fill_kernel(float *p, size_t n, float v)
 {
@@ -164,20 +164,91 @@ __device__ __forceinline__ void grad_piece(const float *__restrict__ src, const 
     reduce_scatter_add<L>(part, row, lane);
 }
 
+// A partials of one kernel for the few pixels of a wave that selected it,
+// with the kernel's taps on the lanes (tap t = 64 i + lane, IT = number of
+// 64-tap slices): every pixel p with coefficient c_p adds c_p x[p + t] to its
+// lane's sums, then each lane adds its taps' sums into the row.  Cost grows
+// with the pixels, not with the wave: one LDS read and one FMA per pixel and
+// tap, against the tile walk's sliding window over all 512 pixels of the wave.
+// The tap offsets in the tile are read from tap_lds, one load per slice.
+template <int IT>
+__device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int base0, int lp,
+                                              const int *__restrict__ tap_lds, int w0, int taps, const float (&m)[GP],
+                                              const int (&sd)[GP], int s, float *__restrict__ row, int lane)
+{
+    int o[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int t = 64 * i + lane;
+        o[i] = t < taps ? as_global(tap_lds)[w0 + t] : 0;
+    }
+    float acc[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) acc[i] = 0.0f;
+    // lane L holds pixels (L & 7) * GP + p of row L >> 3 of the wave
+    auto pixel = [&](int L, int p, float &c) {
+        c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m[p]), L));
+        return base0 + (L >> 3) * lp + (L & 7) * GP + p;
+    };
+#pragma unroll
+    for (int p = 0; p < GP; ++p) {
+        unsigned long long mask = __ballot(sd[p] == s);
+        // two pixels per step while two are left, so their reads overlap
+        while (mask & (mask - 1)) {
+            const int L0 = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const int L1 = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            float c0, c1;
+            const int b0 = pixel(L0, p, c0), b1 = pixel(L1, p, c1);
+            float v0[IT], v1[IT];
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                v0[i] = xt[b0 + o[i]];
+                v1[i] = xt[b1 + o[i]];
+            }
+#pragma unroll
+            for (int i = 0; i < IT; ++i) acc[i] = __builtin_fmaf(c1, v1[i], __builtin_fmaf(c0, v0[i], acc[i]));
+        }
+        if (mask) {
+            float c0;
+            const int b0 = pixel(__builtin_ctzll(mask), p, c0);
+#pragma unroll
+            for (int i = 0; i < IT; ++i) acc[i] = __builtin_fmaf(c0, xt[b0 + o[i]], acc[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int t = 64 * i + lane;
+        if (t < taps) unsafeAtomicAdd(row + w0 + t, acc[i]);
+    }
+}
+
+// The cheaper of the two A walks for one kernel and wave, in measured time
+// units: the tile walk costs about kPieceCost per run piece whatever the
+// pixels; taps on lanes about kLanePixelCost per pixel and 64-tap slice.
+constexpr int kPieceCost = 100, kLanePixelCost = 12, kMaxLaneSlices = 4;
+
 // Pass 1.  For every tile: q = g / sum_d inv_d (written to the q map for pass
-// 2); then, per direction and for every scale selected by some pixel of the
-// wave, one walk over that kernel's runs forms the wave's A partials with the
-// masked coefficients m_p = [s_d(p) = s] q_p inv, and the wave's E partial.
-// yhat is recomputed with the forward's own arithmetic, so it equals the
-// estimate y was built from bit for bit (E is a difference yhat - y and
-// enters dL/dw beside A, where the two can cancel): RING walks the ring sums
-// up to the wave's deepest selected scale, in fixed point when every value of
-// this window is in range (the forward decides the same per window of its own
-// tile, so the two agree unless an out-of-range value lies within reach of
-// one window and not the other); otherwise the same weighted walk that forms
-// A also accumulates yhat.  Each work-group
-// owns one row of `partial` ([grid][NW + K]); its waves add into it with
-// float atomics that return nothing.  No barrier after the tile is staged.
+// 2), then two phases.
+//   yhat and E (RING): yhat of the selected scale is recomputed with the
+//   forward's own arithmetic, so it equals the estimate y was built from bit
+//   for bit (E is a difference yhat - y and enters dL/dw beside A, where the
+//   two can cancel): the ring sums up to the wave's deepest selected scale, in
+//   fixed point from row-prefix tables when every value of this window is in
+//   range (the forward decides the same per window of its own tile, so the
+//   two agree unless an out-of-range value lies within reach of one window and
+//   not the other), else float ring sums over the float tile.  Then E per
+//   scale from the wave.
+//   A: the float tile; per direction and for every scale selected by some
+//   pixel of the wave, the A partials of that kernel with the masked
+//   coefficients m_p = [s_d(p) = s] q_p inv, either by one walk over the
+//   kernel's runs for the wave's 512 pixels (partials summed across the lanes
+//   by a reduce-scatter) or, when few pixels of the wave selected the kernel,
+//   with its taps on the lanes (taps_on_lanes).  Without a ring form the walk
+//   also accumulates yhat, and E follows it.
+// Each work-group owns one row of `partial` ([grid][NW + K]); its waves add
+// into it with float atomics that return nothing.
 template <bool RING>
 __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                                                             const BwdDesc *__restrict__ imgs,
@@ -188,10 +259,12 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
     extern __shared__ __align__(16) float lds_f[];
     __shared__ int next;
     const int S = ks.S, D = ks.D, K = D * S;
-    const int lp = lds_pitch(GW, R);
+    const int lp = lds_pitch(GW, R), lpq = lds_pitch(GW + 1, R);
     float *xt = lds_f;
+    uint32_t *qt = (uint32_t *)lds_f;
     float *tab = lds_f + grad_tab_offset(R);
     const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int NW = adj.num_weights;
     float *__restrict__ row = partial + (size_t)blockIdx.x * (NW + K);
 
@@ -206,11 +279,13 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
         const int x0 = (int)(te.y & 0xffffu) * GW, y0 = (int)(te.y >> 16) * GH;
         for (int i = tid; i < 3 * K; i += kThreads) tab[i] = as_const(img.tab)[i];
         // fixed: the forward's fixed-point ring sums apply to this window
-        // (every value within the range); the test is also the barrier.
+        // (every value within the range); the vote is also the barrier.
         bool fixed = false;
-        if constexpr (RING) {
-            fixed = stage_tile_fits<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R, rings.qrange);
-        } else {
+        if constexpr (RING)
+            fixed = stage_prefix<GW, GH>(qt, lpq, img.x, img.W, img.H, img.pitch, x0, y0, R, rings.qscale,
+                                         rings.qrange);
+        if (!fixed) {
+            __syncthreads();  // every lane has voted before the window is restaged
             stage_tile<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
             __syncthreads();
         }
@@ -238,22 +313,23 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
             if (inside[p]) gq[p] = q[p];
         }
 
-        for (int d = 0; d < D; ++d) {
-            int sd[GP];
-            const auto gmd = gmap + (size_t)d * plane;
+        if constexpr (RING) {
+            // The prefix value Q[.][R + tx*GP] is the left boundary of a run
+            // that starts at the lane's first pixel.
+            const uint32_t *lane_q = qt + (ty + R) * lpq + tx * GP + R;
+            for (int d = 0; d < D; ++d) {
+                int sd[GP];
+                const auto gmd = gmap + (size_t)d * plane;
 #pragma unroll
-            for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? gmd[p] : 255;
-            float yh[GP];
-#pragma unroll
-            for (int p = 0; p < GP; ++p) yh[p] = 0.0f;
-            if constexpr (RING) {
+                for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? gmd[p] : 255;
                 // The forward's ring walk: running sum over rings 0..s up to
                 // the deepest selected scale; yhat = (w_s 2^-F) * (fixed-point
                 // sum), or w_s * (float sum) for a window outside the range.
-                float rs[GP];
+                float yh[GP], rs[GP];
                 uint32_t rq[GP];
 #pragma unroll
                 for (int p = 0; p < GP; ++p) {
+                    yh[p] = 0.0f;
                     rs[p] = 0.0f;
                     rq[p] = 0u;
                 }
@@ -264,21 +340,49 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
                     if (!__any(need)) break;
                     const int k = d * S + s;
                     if (fixed) {
-                        if (need) quantized_runs<GP>(lane_x, lp, rings.runs, load_const(rings.kern + k), rings.qscale, rq);
+                        if (need) prefix_runs<GP>(lane_q, lpq, rings.runs, load_const(rings.kern + k), rq);
                         const float wq = as_const(rings.scale)[k] * rings.qinv;
 #pragma unroll
                         for (int p = 0; p < GP; ++p)
                             if (sd[p] == s) yh[p] = mul_rounded((float)(int)rq[p], wq);
                     } else {
                         if (need)
-                            correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr, load_const(rings.kern + k), rs);
+                            correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr,
+                                                                 load_const(rings.kern + k), rs);
                         const float w = as_const(rings.scale)[k];
 #pragma unroll
                         for (int p = 0; p < GP; ++p)
                             if (sd[p] == s) yh[p] = rs[p] * w;
                     }
                 }
+                // E_k: sum over the pixels that selected s of c (yhat - y).
+                for (int s = 0; s < S; ++s) {
+                    bool want = false;
+#pragma unroll
+                    for (int p = 0; p < GP; ++p) want |= sd[p] == s;
+                    if (!__any(want)) continue;
+                    const int k = d * S + s;
+                    const float inv = inv_t[k];
+                    float e = 0.0f;
+#pragma unroll
+                    for (int p = 0; p < GP; ++p)
+                        if (sd[p] == s) e = __builtin_fmaf(q[p] * inv, yh[p] - yv[p], e);
+                    e = wave_total(e);
+                    if (lane == 0) unsafeAtomicAdd(row + NW + k, e);
+                }
             }
+            if (fixed) {
+                __syncthreads();  // the prefix table is done with
+                stage_tile<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
+                __syncthreads();
+            }
+        }
+
+        for (int d = 0; d < D; ++d) {
+            int sd[GP];
+            const auto gmd = gmap + (size_t)d * plane;
+#pragma unroll
+            for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? gmd[p] : 255;
             for (int s = 0; s < S; ++s) {
                 bool want = false;
 #pragma unroll
@@ -293,6 +397,24 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
                     yacc[p] = 0.0f;
                 }
                 const int2 kr = load_const(ks.kern + k);
+                if constexpr (RING) {
+                    // Few pixels of the wave selected this kernel: taps on lanes.
+                    int n = 0;
+#pragma unroll
+                    for (int p = 0; p < GP; ++p) n += __popcll(__ballot(sd[p] == s));
+                    const int4 kw = load_const(adj.kern_w + k);
+                    const int slices = (kw.y + 63) >> 6;
+                    if (slices <= kMaxLaneSlices && n * slices * kLanePixelCost < kw.z * kPieceCost) {
+                        const int base0 = (wave * (64 / GLX) + R) * lp + R;
+                        switch (slices) {
+                        case 1: taps_on_lanes<1>(xt, base0, lp, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
+                        case 2: taps_on_lanes<2>(xt, base0, lp, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
+                        case 3: taps_on_lanes<3>(xt, base0, lp, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
+                        default: taps_on_lanes<4>(xt, base0, lp, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
+                        }
+                        continue;
+                    }
+                }
                 for (int r = 0; r < kr.y; ++r) {
                     const int4 run = load_const(ks.runs + kr.x + r);
                     for (int j0 = 0; j0 < run.z; j0 += kGradPiece) {
@@ -310,17 +432,14 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
                         }
                     }
                 }
-                // E_k: sum over the pixels that selected s of c (yhat - y).
                 if constexpr (!RING) {
+                    // E_k from the walk's own yhat.
+                    float e = 0.0f;
 #pragma unroll
-                    for (int p = 0; p < GP; ++p)
-                        if (sd[p] == s) yh[p] = yacc[p];
+                    for (int p = 0; p < GP; ++p) e = __builtin_fmaf(m[p], yacc[p] - yv[p], e);
+                    e = wave_total(e);
+                    if (lane == 0) unsafeAtomicAdd(row + NW + k, e);
                 }
-                float e = 0.0f;
-#pragma unroll
-                for (int p = 0; p < GP; ++p) e = __builtin_fmaf(m[p], yh[p] - yv[p], e);
-                e = wave_total(e);
-                if (lane == 0) unsafeAtomicAdd(row + NW + k, e);
             }
         }
         __syncthreads();  // the LDS tile, the table and `next` are reused by the next tile

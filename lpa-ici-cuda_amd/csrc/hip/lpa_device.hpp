@@ -96,17 +96,24 @@ struct BwdDesc {
     float *q;
 };
 
+// Run pieces the adjoint's tap walk is unrolled for (longer runs are walked
+// in pieces of this length).
+constexpr int kGradPiece = 8;
+
 // Extra kernel-set tables the adjoint reads.
 //   flip: the 180-degree rotated runs (-row, -(col + len - 1), len) with
 //         reversed weights, so the transposed correlation reuses correlate()
-//   tap_pos[i] = {row, col} offset of run weight i from the kernel centre
+//   tap_lds[i] = row * lp + col for run weight i at offset (row, col) from the
+//                kernel centre: its offset in the adjoint's staged tile, whose
+//                row pitch is lp = lds_pitch(kBwdTileW, radius)
 //   tap_kernel[i] = kernel index k of run weight i
-//   kern_w[k] = {first run weight, number of run weights} of kernel k
+//   kern_w[k] = {first run weight, number of run weights, pieces, 0} of
+//               kernel k; pieces = sum over its runs of ceil(len / kGradPiece)
 struct DevAdjoint {
     DevKset flip;
-    const int2 *tap_pos;
+    const int *tap_lds;
     const int *tap_kernel;
-    const int2 *kern_w;
+    const int4 *kern_w;
     const float *sumsq;
     int num_weights;
 };

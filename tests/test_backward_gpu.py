@@ -20,7 +20,7 @@ import torch
 
 import lpa_ici
 import oracle
-from conftest import GAMMA, SIGMA, kernels, noisy_synth
+from conftest import GAMMA, GAUSSIAN_M20, SIGMA, kernels, noisy_synth
 from lpa_ici.autograd import lpa_ici as lpa_ici_fn, template_taps
 
 pytestmark = pytest.mark.gpu
@@ -77,6 +77,43 @@ def test_adjoint_matches_fp64_adjoint(ks):
     dk64 = dkd64[idx]
     e_dk = np.abs(dk.cpu().numpy() - dk64).max() / np.abs(dk64).max()
     print(f"adjoint vs FP64: dx rel {e_dx:.2e}, dk rel {e_dk:.2e}")
+    assert e_dx < 1e-4 and e_dk < 1e-4
+
+
+def adjoint_error(ks, x):
+    """Largest deviations of the HIP adjoint from the FP64 adjoint, relative
+    to each gradient's scale."""
+    xd, y, m = forward_gpu(ks, x)
+    g = torch.randn_like(xd)
+    dxs, dk = lpa_ici.backward(ks, [xd], [y], [m], [g], SIGMA, GAMMA)
+    torch.cuda.synchronize()
+    dx64, dkd64 = oracle.adjoint_f64(x, g.cpu().numpy(), ks, SIGMA, m.cpu().numpy())
+    r = ks.runs()
+    dk64 = dkd64[np.array([dense_index(ks, r, i)[1] for i in range(len(r["weights"]))])]
+    e_dx = np.abs(dxs[0].cpu().numpy() - dx64).max() / np.abs(dx64).max()
+    e_dk = np.abs(dk.cpu().numpy() - dk64).max() / np.abs(dk64).max()
+    return e_dx, e_dk
+
+
+def test_adjoint_weighted_kernel_set():
+    """A kernel set with no ring form (Gaussian weights): yhat and E come from
+    the same weighted walk that forms A."""
+    ks_g = kernels(GAUSSIAN_M20)
+    assert not ks_g.info()["ring_form"]
+    e_dx, e_dk = adjoint_error(ks_g, noisy_synth(72, 56, seed=33))
+    print(f"weighted set vs FP64: dx rel {e_dx:.2e}, dk rel {e_dk:.2e}")
+    assert e_dx < 1e-4 and e_dk < 1e-4
+
+
+def test_adjoint_fixed_point_fallback(ks):
+    """Values outside the fixed-point range in one corner: the windows within
+    reach of them recompute yhat with float ring sums, as the forward does,
+    the others from the prefix tables."""
+    x = noisy_synth(200, 120, seed=34)
+    x[4:9, 6:12] = 4.0
+    assert np.abs(x).max() > ks.info()["ring_range"]
+    e_dx, e_dk = adjoint_error(ks, x)
+    print(f"fixed-point fallback vs FP64: dx rel {e_dx:.2e}, dk rel {e_dk:.2e}")
     assert e_dx < 1e-4 and e_dk < 1e-4
 
 
