@@ -148,7 +148,7 @@ typedef struct lpa_tile_config {
     int px_per_lane;      /* adjacent output pixels one lane computes */
     int body;             /* inner loop: 1 = one unrolled body per run length, 0 = sliding-window loop */
     int halo;             /* LDS halo radius, >= the kernel set's max radius */
-    int blocks_per_cu;    /* persistent work-groups per CU (batched form) */
+    int blocks_per_cu;    /* persistent work-groups per CU (batched form) at the full halo; 0 without a device */
 } lpa_tile_config;
 
 /* Number of compiled tile configurations, and the i-th of them. */

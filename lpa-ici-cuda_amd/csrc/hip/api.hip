@@ -148,7 +148,7 @@ void pack_runs(lpa_kset *ks, const float *dense, const int *offsets, const int *
     // Adjoint tables: tap positions and owners, and the rotated runs.
     ks->weights_flip.resize(ks->weights.size());
     ks->tap_lds.resize(ks->weights.size());
-    const int grad_pitch = lpa::lds_pitch(lpa::kBwdTileW, radius);
+    const int grad_pitch = lpa::lds_pitch(lpa::kBwdTileW, radius, lpa::kBwdPitchResidue);
     ks->tap_kernel.resize(ks->weights.size());
     for (int k = 0; k < ks->K; ++k) {
         const int2 kr = ks->kern[k];
@@ -537,7 +537,9 @@ lpa_status lpa_tile_config_get(int index, lpa_tile_config *cfg)
     cfg->px_per_lane = lpa::kShapes[index].px;
     cfg->body = lpa::kShapes[index].body;
     cfg->halo = lpa::kMaxRadius;
-    cfg->blocks_per_cu = 0;
+    int devices = 0;
+    cfg->blocks_per_cu =
+        hipGetDeviceCount(&devices) == hipSuccess && devices > 0 ? lpa::blocks_per_cu(index, lpa::kMaxRadius) : 0;
     return LPA_OK;
 }
 

@@ -34,10 +34,14 @@ constexpr int kMaxScales = 16;
 constexpr int GW = 64, GH = 32, GP = 8, GLX = GW / GP;
 static_assert(GLX * GH == kThreads, "pass-1 tile must use 256 lanes");
 
-// The float tile (pitch lds_pitch(GW, R)) and the prefix table (one more
-// column, pitch lds_pitch(GW + 1, R)) share the start of the LDS; the [3][K]
-// table follows the larger of the two.
-__host__ __device__ inline int grad_tab_offset(int R) { return lds_floats(GW + 1, GH, R); }
+// The float tile (pitch lds_pitch(GW, R, kBwdPitchResidue)) and the prefix
+// table (one more column, pitch lds_pitch(GW + 1, R)) share the start of the
+// LDS; the [3][K] table follows the larger of the two.
+__host__ __device__ inline int grad_tab_offset(int R)
+{
+    const int a = lds_floats(GW, GH, R, kBwdPitchResidue), b = lds_floats(GW + 1, GH, R);
+    return a > b ? a : b;
+}
 
 __global__ __launch_bounds__(kThreads) void fill_kernel(float *p, size_t n, float v)
 {
@@ -185,10 +189,10 @@ __device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int 
     float acc[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) acc[i] = 0.0f;
-    // lane L holds pixels (L & 7) * GP + p of row L >> 3 of the wave
+    // lane L holds pixels (L % GLX) * GP + p of row L / GLX of the wave
     auto pixel = [&](int L, int p, float &c) {
         c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m[p]), L));
-        return base0 + (L >> 3) * lp + (L & 7) * GP + p;
+        return base0 + (L / GLX) * lp + (L % GLX) * GP + p;
     };
 #pragma unroll
     for (int p = 0; p < GP; ++p) {
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
     extern __shared__ __align__(16) float lds_f[];
     __shared__ int next;
     const int S = ks.S, D = ks.D, K = D * S;
-    const int lp = lds_pitch(GW, R), lpq = lds_pitch(GW + 1, R);
+    const int lp = lds_pitch(GW, R, kBwdPitchResidue), lpq = lds_pitch(GW + 1, R);
     float *xt = lds_f;
     uint32_t *qt = (uint32_t *)lds_f;
     float *tab = lds_f + grad_tab_offset(R);
@@ -452,6 +456,7 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
 // the table.  The wide tile cuts the masked-map building per output pixel
 // (halo overhead (64+2R)^2/64^2 instead of (32+2R)^2/32^2).
 constexpr int DXW = 64, DXH = 64, DXP = 4, DXT = 1024, DXLX = DXW / DXP;
+static_assert(DXLX == 16, "the lane layout assumes 16 lanes per tile row");
 constexpr int kSlotRows = 8, kSlotCols = 2;  // 16 + 16i rows, 64 + 64j columns cover 126 x 126
 
 //
@@ -499,7 +504,10 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings,
                 qv[i][j] = in ? as_global(bd.q)[(size_t)gy * img.W + gx] : 0.0f;
             }
 
-        const int tx = tid % DXLX, ty = tid / DXLX;
+        // A wave covers 4 rows; each half-wave a 32-pixel-wide half of them
+        // (8 lanes per row, 4 px apart, rows one bank apart: 32 banks).
+        const int dl = tid & 63;
+        const int tx = (dl >> 5) * 8 + (dl & 7), ty = (tid >> 6) * 4 + ((dl & 31) >> 3);
         const int px0 = x0 + tx * DXP, py = y0 + ty;
         const float *lane_m = mt + (ty + R) * lp + tx * DXP + R;
         float dx[DXP];

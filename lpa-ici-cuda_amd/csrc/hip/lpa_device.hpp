@@ -105,7 +105,7 @@ constexpr int kGradPiece = 8;
 //         reversed weights, so the transposed correlation reuses correlate()
 //   tap_lds[i] = row * lp + col for run weight i at offset (row, col) from the
 //                kernel centre: its offset in the adjoint's staged tile, whose
-//                row pitch is lp = lds_pitch(kBwdTileW, radius)
+//                row pitch is lp = lds_pitch(kBwdTileW, radius, kBwdPitchResidue)
 //   tap_kernel[i] = kernel index k of run weight i
 //   kern_w[k] = {first run weight, number of run weights, pieces, 0} of
 //               kernel k; pieces = sum over its runs of ceil(len / kGradPiece)
@@ -146,6 +146,11 @@ hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &ri
 //            per work-group of the first pass, accumulated by its waves
 //   dkernel: += reduction of the rows plus the variance term (may be null)
 constexpr int kBwdTileW = 64, kBwdTileH = 32, kDxTile = 64;
+// Row pitch residue (mod 32 banks) of the first pass's float tile: with rows
+// 9 banks apart, the 64 taps a wave reads together when the taps of a kernel
+// are on the lanes spread over the banks for every direction of the shipped
+// sets about twice as well as with rows one bank apart.
+constexpr int kBwdPitchResidue = 9;
 //   rings:   the ring form (runs == nullptr when the set has none); the
 //            forward's default walk, which the adjoint repeats for yhat
 hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAdjoint &adj, const BwdDesc *d_imgs,
