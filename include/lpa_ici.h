@@ -202,6 +202,25 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image* imgs, const float* c
 lpa_status lpa_load_kernels(const char* path, int* num_dirs, int* num_scales, int* n_weights, float* weights,
                             int* offsets, int* rows, int* cols);
 
+/* Kernel-set generation (no counterpart in the reference's code: its sets
+ * come precomputed from a modified LASIP demo_CreateLPAKernels.m that is not
+ * in the repository, readme.md:87-92,239).  Uniform wedges: kernel (d, s) is
+ * 1/n (six decimals, as the files store it) on the n lattice points (x, y)
+ * (y = -row) that are the centre or lie within pi/D of direction
+ * phi_d = 2 pi d / D and project onto it below h[s] - 1/2; (2h-1)^2 dense,
+ * centred.  D = 16 with h = 1,2,4,8,16,24,32 gives the shipped default set
+ * (lpa_kernels_m_1_0_d_16_h12_1_2_4_8_16_24_32_symmetric.txt) weight for
+ * weight.  Needs D >= 4 and 1 <= h <= 32.  Same two-call protocol and
+ * layout as lpa_load_kernels. */
+lpa_status lpa_wedge_kernels(int num_dirs, int num_scales, const int* h, int* n_weights, float* weights,
+                             int* offsets, int* rows, int* cols);
+
+/* Write a kernel set in the reference's text format (the format
+ * lpa_load_kernels reads): "D S", then per kernel "rows cols" and its rows
+ * of "%f" weights.  Errors: LPA_EINVAL, LPA_EIO. */
+lpa_status lpa_write_kernels(const char* path, int num_dirs, int num_scales, const float* weights,
+                             const int* offsets, const int* rows, const int* cols);
+
 /* compute_kernel_variances, src/v0_0_lpa_ici_2D_naive.cpp:675-701:
  * var[k] = (sigma*sigma) * sum(w^2), float accumulation in dense order. */
 void lpa_kernel_variances(int num_kernels, const float* weights, const int* offsets, const int* rows,

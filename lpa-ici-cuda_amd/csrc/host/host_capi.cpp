@@ -46,6 +46,34 @@ lpa_status lpa_load_kernels(const char *path, int *num_dirs, int *num_scales, in
     return LPA_OK;
 }
 
+lpa_status lpa_wedge_kernels(int num_dirs, int num_scales, const int *h, int *n_weights, float *weights,
+                             int *offsets, int *rows, int *cols)
+{
+    if (!n_weights) return lpa::fail(LPA_EINVAL, "null argument");
+    std::vector<float> w;
+    std::vector<int> off, r, c;
+    std::string err = lpa::wedge_kernels(num_dirs, num_scales, h, w, off, r, c);
+    if (!err.empty()) return lpa::fail(LPA_EINVAL, err);
+    *n_weights = (int)w.size();
+    if (weights) std::memcpy(weights, w.data(), w.size() * sizeof(float));
+    if (offsets) std::memcpy(offsets, off.data(), off.size() * sizeof(int));
+    if (rows) std::memcpy(rows, r.data(), r.size() * sizeof(int));
+    if (cols) std::memcpy(cols, c.data(), c.size() * sizeof(int));
+    lpa::clear_error();
+    return LPA_OK;
+}
+
+lpa_status lpa_write_kernels(const char *path, int num_dirs, int num_scales, const float *weights,
+                             const int *offsets, const int *rows, const int *cols)
+{
+    if (!path || !weights || !offsets || !rows || !cols || num_dirs < 1 || num_scales < 1)
+        return lpa::fail(LPA_EINVAL, "null argument or empty kernel set");
+    std::string err = lpa::write_kernels(path, num_dirs, num_scales, weights, offsets, rows, cols);
+    if (!err.empty()) return lpa::fail(LPA_EIO, err);
+    lpa::clear_error();
+    return LPA_OK;
+}
+
 // var_k = (sigma*sigma) * sum(w^2): the square of sigma is rounded to float
 // once, the sum runs in float over the dense kernel in row-major order.
 void lpa_kernel_variances(int num_kernels, const float *weights, const int *offsets, const int *rows,

@@ -81,6 +81,16 @@ bool write_hdr(const std::string &path, int width, int height, int channels, con
 // Deterministic synthetic image in [0,1].
 void synth_image(float *out, int width, int height, unsigned seed);
 
+// Uniform wedge kernel set (kernel_gen.cpp): D directions, S window lengths
+// h[0..S); dense (2h-1)^2 kernels in the loader's layout.  "" or an error.
+std::string wedge_kernels(int D, int S, const int *h, std::vector<float> &w, std::vector<int> &off,
+                          std::vector<int> &rows, std::vector<int> &cols);
+
+// The reference's kernel text format: "D S", then per kernel "rows cols" and
+// its rows of "%f" weights.  "" or an error.
+std::string write_kernels(const std::string &path, int D, int S, const float *w, const int *off, const int *rows,
+                          const int *cols);
+
 }  // namespace lpa
 
 #endif

@@ -16,7 +16,8 @@ from ._lib import (DEFAULT_KERNELS, FWD_EXACT, FWD_GRAPH, FWD_PLAIN, FWD_RING_FL
 __all__ = [
     "DEFAULT_KERNELS", "GAUSSIAN_M20_KERNELS", "KERNEL_DIR", "LIB_PATH", "FWD_PLAIN", "FWD_EXACT", "FWD_GRAPH",
     "FWD_WEIGHTED", "FWD_RING_FLOAT",
-    "LpaError", "KernelSet", "Plan", "load_kernels", "kernel_variances", "make_noisy", "psnr", "synth_image",
+    "LpaError", "KernelSet", "Plan", "load_kernels", "wedge_kernels", "write_kernels", "kernel_variances",
+    "make_noisy", "psnr", "synth_image",
     "load_image", "write_hdr", "denoise", "estimates", "backward", "image_struct", "tile_configs", "lib",
 ]
 
@@ -149,6 +150,26 @@ def load_kernels(path=DEFAULT_KERNELS):
     check(lib().lpa_load_kernels(p, ctypes.byref(D), ctypes.byref(S), ctypes.byref(nw), _ptr(w), _ptr(off),
                                  _ptr(rows), _ptr(cols)))
     return KernelSet(D.value, S.value, w, off, rows, cols, path=path)
+
+
+def wedge_kernels(num_dirs, h):
+    """Uniform wedge kernel set: num_dirs directions, window lengths h (one
+    scale each); see lpa_wedge_kernels in include/lpa_ici.h."""
+    hs = _i32(h)
+    nw = ctypes.c_int()
+    check(lib().lpa_wedge_kernels(int(num_dirs), hs.size, _ptr(hs), ctypes.byref(nw), None, None, None, None))
+    K = int(num_dirs) * hs.size
+    w = np.zeros(nw.value, np.float32)
+    off, rows, cols = (np.zeros(K, np.int32) for _ in range(3))
+    check(lib().lpa_wedge_kernels(int(num_dirs), hs.size, _ptr(hs), ctypes.byref(nw), _ptr(w), _ptr(off),
+                                  _ptr(rows), _ptr(cols)))
+    return KernelSet(num_dirs, hs.size, w, off, rows, cols)
+
+
+def write_kernels(path, ks):
+    """Write a kernel set in the reference's text format (load_kernels reads it back)."""
+    check(lib().lpa_write_kernels(str(path).encode(), ks.D, ks.S, _ptr(ks.weights), _ptr(ks.offsets),
+                                  _ptr(ks.rows), _ptr(ks.cols)))
 
 
 def kernel_variances(ks, sigma):

@@ -132,6 +132,26 @@ def test_fixed_point_bitexact(ks, w, h):
     assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
 
 
+@pytest.mark.parametrize("D,h", [(8, [1, 2, 4, 8, 16]), (32, [1, 3, 6, 12, 24, 32])])
+def test_generated_wedge_sets(D, h):
+    """Generated uniform-wedge sets with other direction counts and scales:
+    exact mode equals the oracle's v0_0 bits, the default walk (fixed-point
+    ring sums) the oracle's fixed-point restatement, output and maps."""
+    g = lpa_ici.wedge_kernels(D, h)
+    assert g.info()["ring_form"] == 1
+    x = noisy_synth(160, 96, seed=D)
+    var = g.variances(SIGMA)
+    y, maps = run(g, x, flags=lpa_ici.FWD_EXACT)
+    yo, mo, _ = oracle.walk_f32(x, g, var, GAMMA)
+    assert np.array_equal(maps, mo)
+    assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
+    F, rng = fixed_point(g)
+    y, maps = run(g, x)
+    yq, mq = oracle.walk_q(x, g, var, GAMMA, F, rng)
+    assert np.array_equal(maps, mq)
+    assert np.array_equal(y.view(np.uint32), yq.view(np.uint32))
+
+
 def test_fixed_point_fallback_windows(ks):
     """A block of values outside the fixed-point range: windows that reach it
     take float ring sums, the rest stay in fixed point.  Far from the block
