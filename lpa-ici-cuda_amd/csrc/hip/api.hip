@@ -83,7 +83,9 @@ struct lpa_kset {
 
 namespace {
 
-constexpr int kDefaultShape = 0;
+// Tile configuration used until one is set (lpa_kset_set_tile_config): the
+// fastest on a 4096^2 image of the default set (1024 lanes, 64 x 64 tiles).
+constexpr int kDefaultShape = 9;
 
 lpa_status hip_fail(hipError_t e, const char *what)
 {
@@ -414,7 +416,7 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
     std::memcpy(host.data() + off_tiles, tiles.data(), sizeof(uint2) * tiles.size());
     HIP_TRY(hipMemcpyAsync(p->d_mem, host.data(), bytes, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));  // `host` goes out of scope on return
-    const int bpc = lpa::blocks_per_cu(p->shape, ks->radius);
+    const int bpc = lpa::blocks_per_cu(p->shape, ks->radius, ks->K);
     p->grid = std::max(1, std::min(p->num_tiles, num_cus() * bpc));
     return LPA_OK;
 }
@@ -539,7 +541,7 @@ lpa_status lpa_tile_config_get(int index, lpa_tile_config *cfg)
     cfg->halo = lpa::kMaxRadius;
     int devices = 0;
     cfg->blocks_per_cu =
-        hipGetDeviceCount(&devices) == hipSuccess && devices > 0 ? lpa::blocks_per_cu(index, lpa::kMaxRadius) : 0;
+        hipGetDeviceCount(&devices) == hipSuccess && devices > 0 ? lpa::blocks_per_cu(index, lpa::kMaxRadius, 0) : 0;
     return LPA_OK;
 }
 
@@ -591,7 +593,7 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
         for (int y = 0; y < ty; ++y)
             for (int x = 0; x < tx; ++x) tiles.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
     }
-    const int grid = std::max(1, std::min((int)tiles.size(), num_cus() * lpa::blocks_per_cu(SH, ks->radius)));
+    const int grid = std::max(1, std::min((int)tiles.size(), num_cus() * lpa::blocks_per_cu(SH, ks->radius, ks->K)));
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t off_desc = 256;
     const size_t off_tiles = off_desc + up(sizeof(lpa::EstDesc) * n);

@@ -54,24 +54,33 @@ struct ImgDesc {
 };
 
 // Compiled tile shapes: output tile tw x th pixels, px adjacent pixels per
-// lane along a row, 256 lanes per work-group, the inner-loop body
-// (1: length-dispatched unrolled runs, 0: sliding-window loop) and the LDS
-// row pitch's residue mod 32 (the bank offset between consecutive rows).
+// lane along a row, nt lanes per work-group, the inner-loop body (1:
+// length-dispatched unrolled runs, 0: sliding-window loop), the LDS row
+// pitch's residue mod 32 (the bank offset between consecutive rows), the
+// lane layout (split 1: each half-wave, whose 32 lanes the LDS serves
+// together, takes its own half of the rows' lanes, so the 32 lanes of a read
+// sit 8 or 4 px apart on rows one bank apart: 32 distinct banks) and the
+// waves per SIMD the register budget is held to (wpe; 1: no limit).
 struct TileShape {
-    int tw, th, px, body, pm;
+    int tw, th, px, body, pm, nt, split, wpe;
 };
 constexpr int kThreads = 256;
-constexpr int kNumShapes = 7;
+constexpr int kNumShapes = 10;
 constexpr TileShape kShapes[kNumShapes] = {
-    {32, 32, 4, 1, 1},  // 0: wave covers 32x8 px
-    {32, 32, 4, 0, 1},  // 1: same, window loop
-    {64, 16, 4, 1, 1},  // 2: wave covers 64x4 px
-    {32, 16, 2, 1, 1},  // 3: wave covers 32x4 px
-    {64, 32, 8, 1, 1},  // 4: wave covers 64x8 px
-    {32, 64, 8, 1, 1},  // 5: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
-    {32, 64, 8, 1, 2},  // 6: as 5, rows two banks apart: the dword pairs of 16 lanes (4 rows) are disjoint
+    {32, 32, 4, 1, 1, 256, 0, 1},   // 0: wave covers 32x8 px
+    {32, 32, 4, 0, 1, 256, 0, 1},   // 1: same, window loop
+    {64, 16, 4, 1, 1, 256, 0, 1},   // 2: wave covers 64x4 px
+    {32, 16, 2, 1, 1, 256, 0, 1},   // 3: wave covers 32x4 px
+    {64, 32, 8, 1, 1, 256, 0, 1},   // 4: wave covers 64x8 px
+    {32, 64, 8, 1, 1, 256, 0, 1},   // 5: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
+    {32, 64, 8, 1, 2, 256, 0, 1},   // 6: as 5, rows two banks apart: the dword pairs of 16 lanes (4 rows) are disjoint
+    {64, 64, 8, 1, 1, 512, 1, 4},   // 7: halo 3.9 words per output pixel against 5.8; two work-groups per CU
+    {64, 32, 4, 1, 1, 512, 1, 6},   // 8: three work-groups of 512 lanes per CU (24 waves)
+    {64, 64, 4, 1, 1, 1024, 1, 8},  // 9: 1024 lanes within 64 VGPRs: two work-groups per CU (32 waves)
 };
 constexpr int kMaxRadius = 31;
+// LDS of one work-group on gfx950.
+constexpr int kMaxLdsBytes = 160 * 1024;
 
 // LDS row pitch for a tile of width tw and halo radius r: at least tw + 2r and
 // congruent to pm mod 32 (default 1: lanes four pixels apart on consecutive
@@ -124,8 +133,8 @@ hipError_t launch_forward_plain(int shape, WalkMode mode, const DevKset &ks, con
 hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, const DevRings &rings,
                                   const ImgDesc *d_imgs, const uint2 *d_tiles, int num_tiles, int *d_counter,
                                   int radius, int grid, hipStream_t stream);
-// Work-groups of one shape that fit on a CU at this radius (LDS bound).
-int blocks_per_cu(int shape, int radius);
+// Work-groups of one shape that fit on a CU at this radius and kernel count.
+int blocks_per_cu(int shape, int radius, int num_kernels);
 
 // Per-direction estimates at frozen scales (lpa_fwd.hip): est = [D][H][W]
 // floats per image, dense like the scale map.  Tile configuration

@@ -28,13 +28,13 @@ namespace {
 // the prefix value at its first pixel's column (kWalkRing, fixed point).
 template <int P, int MODE, int BODY, typename T>
 __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, const DevKset &ks, const DevRings &rings,
-                                          const float *__restrict__ tab, float *__restrict__ y,
+                                          const float *__restrict__ tab, const float *__restrict__ inv_lds,
+                                          float *__restrict__ y,
                                           uint8_t *__restrict__ map, int W, int H, int pitch, int px0, int py)
 {
     constexpr bool EXACT = MODE == kWalkExact;
     const int S = ks.S, K = ks.D * ks.S;
     const float *__restrict__ thr_t = tab + K;
-    const float *__restrict__ inv_t = tab + 2 * K;
 
     bool inside[P];
     float num[P], den[P];
@@ -48,8 +48,8 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
     const size_t pix = (size_t)py * W + px0;
 
     for (int d = 0; d < ks.D; ++d) {
-        float lo[P], hi[P], best[P], binv[P];
-        int sel[P];
+        float lo[P], hi[P], best[P];
+        int sel[P];  // -1 until a scale's interval is non-empty
         float ring_sum[P];    // float ring sums: running sum over scales 0..s
         uint32_t ring_q[P];   // fixed-point ring sums
 #pragma unroll
@@ -60,8 +60,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
             lo[p] = inside[p] ? -INFINITY : INFINITY;
             hi[p] = inside[p] ? INFINITY : -INFINITY;
             best[p] = 0.0f;
-            binv[p] = 0.0f;
-            sel[p] = 0;
+            sel[p] = -1;
         }
         // alive: some pixel of the lane still has a non-empty interval (it
         // is the `ok` of the previous scale; an empty intersection stays
@@ -92,7 +91,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 for (int p = 0; p < P; ++p) acc[p] = 0.0f;
                 if (alive) correlate<P, EXACT, BODY>(src, lp, ks, k, acc);
             }
-            const float thr = as_const(thr_t)[k], inv = as_const(inv_t)[k];
+            const float thr = as_const(thr_t)[k];
             bool any_ok = false;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
@@ -103,7 +102,6 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 const bool ok = h >= l;
                 if (ok) {
                     best[p] = acc[p];
-                    binv[p] = inv;
                     sel[p] = s;
                 }
                 any_ok |= ok;
@@ -112,14 +110,18 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            num[p] = mul_add_rounded(num[p], best[p], binv[p]);
-            den[p] = den[p] + binv[p];
+            // 1/var of the selected scale, read once per direction (the
+            // intersection only shrinks, so the last scale with a non-empty
+            // interval is the selected one and the walk kept its index)
+            const float binv = sel[p] >= 0 ? inv_lds[d * S + sel[p]] : 0.0f;
+            num[p] = mul_add_rounded(num[p], best[p], binv);
+            den[p] = den[p] + binv;
         }
         if (map) {
             const auto gmap = as_global(map) + (size_t)d * plane + pix;
 #pragma unroll
             for (int p = 0; p < P; ++p)
-                if (inside[p]) gmap[p] = (uint8_t)sel[p];
+                if (inside[p]) gmap[p] = (uint8_t)max(sel[p], 0);
         }
     }
     const auto yrow = as_global(y) + (size_t)py * pitch + px0;
@@ -135,39 +137,50 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
 // the window as fixed-point prefix sums and walks it if every value lies
 // within the kernel set's range; otherwise, and for the other modes, the
 // window is staged as floats.  Every barrier is reached by every lane.
-template <int TW, int TH, int P, int MODE, int BODY, int PM>
+template <int TW, int TH, int P, int MODE, int BODY, int PM, int NT, int SPLIT>
 __device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks, const DevRings &rings,
                                              const float *__restrict__ tab, const float *__restrict__ x,
                                              float *__restrict__ y, uint8_t *__restrict__ map, int W, int H,
                                              int pitch, int x0, int y0, int R)
 {
     constexpr int LX = TW / P;
-    static_assert(LX * TH == kThreads, "tile shape must use 256 lanes");
+    static_assert(LX * TH == NT, "tile shape must use every lane of the work-group");
+    static_assert(LX >= 2 && (LX & (LX - 1)) == 0 && LX <= 64, "lanes per row: a power of two up to 64");
     const int lp = lds_pitch(TW + 1, R, PM);
     const int tid = threadIdx.x;
-    const int tx = tid % LX, ty = tid / LX;
+    // SPLIT: a wave covers 64 / LX rows; lanes 0-31 take the left LX / 2 lane
+    // columns of them, lanes 32-63 the right LX / 2.
+    constexpr int HW = LX / 2, RW = 64 / LX;
+    const int tx = SPLIT ? ((tid & 32) ? HW : 0) + (tid & (HW - 1)) : tid % LX;
+    const int ty = SPLIT ? (tid >> 6) * RW + (tid & 31) / HW : tid / LX;
     const int px0 = x0 + tx * P, py = y0 + ty;
     // LDS offset of window column R + tx*P of row ty + R, not counting the
     // leading column.  Past the leading column (the float tile) it is the
     // lane's first pixel; in the prefix table it is Q[.][R + tx*P], the left
     // boundary of a run that starts at that pixel.
     const int lane_off = (ty + R) * lp + tx * P + R;
+    // 1/var per kernel, after the window: read once per direction at the
+    // selected scale (the staging barrier orders these writes)
+    const int K = ks.D * ks.S;
+    float *inv_lds = tile + lds_floats(TW + 1, TH, R, PM);
+    for (int i = tid; i < K; i += NT) inv_lds[i] = as_const(tab)[2 * K + i];
     if constexpr (MODE == kWalkRing) {
         uint32_t *q = (uint32_t *)tile;
-        if (stage_prefix<TW, TH>(q, lp, x, W, H, pitch, x0, y0, R, rings.qscale, rings.qrange)) {
-            walk_lane<P, kWalkRing, BODY>(q + lane_off, lp, ks, rings, tab, y, map, W, H, pitch, px0, py);
+        if (stage_prefix<TW, TH, NT>(q, lp, x, W, H, pitch, x0, y0, R, rings.qscale, rings.qrange)) {
+            walk_lane<P, kWalkRing, BODY>(q + lane_off, lp, ks, rings, tab, inv_lds, y, map, W, H, pitch, px0, py);
             return;
         }
         // A value of the window lies outside the fixed-point range (or is
         // not finite): float ring sums.  Every lane takes this branch.
-        stage_tile<TW, TH>(tile + 1, lp, x, W, H, pitch, x0, y0, R);
+        stage_tile<TW, TH, NT>(tile + 1, lp, x, W, H, pitch, x0, y0, R);
         __syncthreads();
-        walk_lane<P, kWalkRingFloat, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, y, map, W, H, pitch, px0, py);
+        walk_lane<P, kWalkRingFloat, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, inv_lds, y, map, W, H, pitch, px0,
+                                           py);
     } else {
         // Pixels outside the image read as zero: the reference's zero padding.
-        stage_tile<TW, TH>(tile + 1, lp, x, W, H, pitch, x0, y0, R);
+        stage_tile<TW, TH, NT>(tile + 1, lp, x, W, H, pitch, x0, y0, R);
         __syncthreads();
-        walk_lane<P, MODE, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, y, map, W, H, pitch, px0, py);
+        walk_lane<P, MODE, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, inv_lds, y, map, W, H, pitch, px0, py);
     }
 }
 
@@ -288,8 +301,8 @@ __global__ __launch_bounds__(kThreads) void estimate_kernel(DevKset ks, DevRings
 // turn, so block b runs tile (b % 8) * per_xcd + b / 8: consecutive tiles of
 // one XCD are neighbours and share their halos in that XCD's L2.
 template <int SH, int MODE>
-__global__ __launch_bounds__(kThreads) void fwd_plain_kernel(DevKset ks, DevRings rings, ImgDesc img, int R,
-                                                             int num_tiles, int per_xcd)
+__global__ __launch_bounds__(kShapes[SH].nt) __attribute__((amdgpu_waves_per_eu(kShapes[SH].wpe)))
+void fwd_plain_kernel(DevKset ks, DevRings rings, ImgDesc img, int R, int num_tiles, int per_xcd)
 {
     extern __shared__ float tile[];
     constexpr TileShape sh = kShapes[SH];
@@ -297,18 +310,17 @@ __global__ __launch_bounds__(kThreads) void fwd_plain_kernel(DevKset ks, DevRing
     const int t = (b & 7) * per_xcd + (b >> 3);
     if (t >= num_tiles) return;  // whole work-group, before any barrier
     const int tx = t % img.tiles_x, ty = t / img.tiles_x;
-    process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W, img.H,
-                                                    img.pitch, tx * sh.tw, ty * sh.th, R);
+    process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm, sh.nt, sh.split>(
+        tile, ks, rings, img.tab, img.x, img.y, img.map, img.W, img.H, img.pitch, tx * sh.tw, ty * sh.th, R);
 }
 
 // Batched form: a persistent grid drains one tile queue that spans every
 // image of the batch.  tiles[t] = {image, (tile_y << 16) | tile_x}.  Every
 // work-group leaves the loop once the counter passes num_tiles.
 template <int SH, int MODE>
-__global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, DevRings rings,
-                                                               const ImgDesc *__restrict__ imgs,
-                                                               const uint2 *__restrict__ tiles, int num_tiles,
-                                                               int *__restrict__ counter, int R)
+__global__ __launch_bounds__(kShapes[SH].nt) __attribute__((amdgpu_waves_per_eu(kShapes[SH].wpe)))
+void fwd_batched_kernel(DevKset ks, DevRings rings, const ImgDesc *__restrict__ imgs,
+                        const uint2 *__restrict__ tiles, int num_tiles, int *__restrict__ counter, int R)
 {
     extern __shared__ float tile[];
     __shared__ int next;
@@ -320,14 +332,31 @@ __global__ __launch_bounds__(kThreads) void fwd_batched_kernel(DevKset ks, DevRi
         if (t >= num_tiles) break;
         const uint2 e = load_const(tiles + t);
         const ImgDesc img = load_const_struct(imgs + e.x);
-        process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm>(tile, ks, rings, img.tab, img.x, img.y, img.map, img.W,
-                                                        img.H, img.pitch, (int)(e.y & 0xffffu) * sh.tw,
-                                                        (int)(e.y >> 16) * sh.th, R);
+        process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm, sh.nt, sh.split>(
+            tile, ks, rings, img.tab, img.x, img.y, img.map, img.W, img.H, img.pitch, (int)(e.y & 0xffffu) * sh.tw,
+            (int)(e.y >> 16) * sh.th, R);
         __syncthreads();  // the LDS tile and `next` are reused
     }
 }
 
 __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
+
+// LDS of one work-group: the window (with its leading column) and the
+// [K] table of 1/var.
+size_t fwd_lds_bytes(const TileShape &sh, int R, int K)
+{
+    return sizeof(float) * ((size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm) + (size_t)K);
+}
+
+// Launches that need more LDS than the default limit raise it first.
+template <typename F>
+hipError_t allow_lds(F *kernel, size_t lds)
+{
+    if (lds <= 65536) return hipSuccess;
+    if (lds > (size_t)kMaxLdsBytes) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds);
+}
 
 template <int SH, int MODE>
 hipError_t plain_launch(const DevKset &ks, const DevRings &rings, const ImgDesc &img, int R, hipStream_t stream)
@@ -336,9 +365,11 @@ hipError_t plain_launch(const DevKset &ks, const DevRings &rings, const ImgDesc 
     const int tiles_y = (img.H + sh.th - 1) / sh.th;
     const int num_tiles = img.tiles_x * tiles_y;
     const int per_xcd = (num_tiles + 7) / 8;
-    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm);
-    hipLaunchKernelGGL((fwd_plain_kernel<SH, MODE>), dim3(8 * per_xcd), dim3(kThreads), lds, stream, ks, rings, img,
-                       R, num_tiles, per_xcd);
+    const size_t lds = fwd_lds_bytes(sh, R, ks.D * ks.S);
+    const hipError_t e = allow_lds(fwd_plain_kernel<SH, MODE>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((fwd_plain_kernel<SH, MODE>), dim3(8 * per_xcd), dim3(sh.nt), lds, stream, ks, rings, img, R,
+                       num_tiles, per_xcd);
     return hipGetLastError();
 }
 
@@ -359,8 +390,10 @@ hipError_t batched_launch(const DevKset &ks, const DevRings &rings, const ImgDes
                           int num_tiles, int *counter, int R, int grid, hipStream_t stream)
 {
     constexpr TileShape sh = kShapes[SH];
-    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm);
-    hipLaunchKernelGGL((fwd_batched_kernel<SH, MODE>), dim3(grid), dim3(kThreads), lds, stream, ks, rings, imgs, tiles,
+    const size_t lds = fwd_lds_bytes(sh, R, ks.D * ks.S);
+    const hipError_t e = allow_lds(fwd_batched_kernel<SH, MODE>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((fwd_batched_kernel<SH, MODE>), dim3(grid), dim3(sh.nt), lds, stream, ks, rings, imgs, tiles,
                        num_tiles, counter, R);
     return hipGetLastError();
 }
@@ -380,11 +413,12 @@ hipError_t batched_dispatch(WalkMode mode, const DevKset &ks, const DevRings &ri
 }
 
 template <int SH>
-int occupancy(int R)
+int occupancy(int R, int K)
 {
-    const size_t lds = sizeof(float) * (size_t)lds_floats(kShapes[SH].tw + 1, kShapes[SH].th, R, kShapes[SH].pm);
+    const size_t lds = fwd_lds_bytes(kShapes[SH], R, K);
+    if (allow_lds(fwd_batched_kernel<SH, kWalkRing>, lds) != hipSuccess) return 1;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, kWalkRing>, kThreads, lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, kWalkRing>, kShapes[SH].nt, lds) !=
         hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
@@ -403,6 +437,9 @@ hipError_t launch_forward_plain(int shape, WalkMode mode, const DevKset &ks, con
     case 4: return plain_dispatch<4>(mode, ks, rings, img, radius, stream);
     case 5: return plain_dispatch<5>(mode, ks, rings, img, radius, stream);
     case 6: return plain_dispatch<6>(mode, ks, rings, img, radius, stream);
+    case 7: return plain_dispatch<7>(mode, ks, rings, img, radius, stream);
+    case 8: return plain_dispatch<8>(mode, ks, rings, img, radius, stream);
+    case 9: return plain_dispatch<9>(mode, ks, rings, img, radius, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -414,7 +451,7 @@ hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, c
     switch (shape) {
 #define LPA_B(SH)                                                                                                  \
     case SH: return batched_dispatch<SH>(mode, ks, rings, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
-        LPA_B(0) LPA_B(1) LPA_B(2) LPA_B(3) LPA_B(4) LPA_B(5) LPA_B(6)
+        LPA_B(0) LPA_B(1) LPA_B(2) LPA_B(3) LPA_B(4) LPA_B(5) LPA_B(6) LPA_B(7) LPA_B(8) LPA_B(9)
 #undef LPA_B
     default: return hipErrorInvalidValue;
     }
@@ -441,16 +478,19 @@ hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &ri
     return hipGetLastError();
 }
 
-int blocks_per_cu(int shape, int radius)
+int blocks_per_cu(int shape, int radius, int num_kernels)
 {
     switch (shape) {
-    case 0: return occupancy<0>(radius);
-    case 1: return occupancy<1>(radius);
-    case 2: return occupancy<2>(radius);
-    case 3: return occupancy<3>(radius);
-    case 4: return occupancy<4>(radius);
-    case 5: return occupancy<5>(radius);
-    case 6: return occupancy<6>(radius);
+    case 0: return occupancy<0>(radius, num_kernels);
+    case 1: return occupancy<1>(radius, num_kernels);
+    case 2: return occupancy<2>(radius, num_kernels);
+    case 3: return occupancy<3>(radius, num_kernels);
+    case 4: return occupancy<4>(radius, num_kernels);
+    case 5: return occupancy<5>(radius, num_kernels);
+    case 6: return occupancy<6>(radius, num_kernels);
+    case 7: return occupancy<7>(radius, num_kernels);
+    case 8: return occupancy<8>(radius, num_kernels);
+    case 9: return occupancy<9>(radius, num_kernels);
     default: return 1;
     }
 }

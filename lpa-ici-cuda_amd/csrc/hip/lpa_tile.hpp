@@ -203,14 +203,14 @@ __device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp
 // Stage the (TW + 2R) x (TH + 2R) window whose top-left image pixel is
 // (x0 - R, y0 - R) into `tile` (row pitch lp); pixels outside the image read
 // as zero.  Rows of 64 lanes read 64 consecutive floats.  No barrier.
-template <int TW, int TH>
+template <int TW, int TH, int NT = kThreads>
 __device__ __forceinline__ void stage_tile(float *__restrict__ tile, int lp, const float *__restrict__ x, int W, int H,
                                            int pitch, int x0, int y0, int R)
 {
     const int tid = threadIdx.x;
     const int fw = TW + 2 * R, fh = TH + 2 * R;
     const auto gx = as_global(x);
-    for (int r = tid >> 6; r < fh; r += kThreads / 64) {
+    for (int r = tid >> 6; r < fh; r += NT / 64) {
         const int gy = y0 - R + r;
         const bool row_in = gy >= 0 && gy < H;
         const auto xrow = gx + (size_t)(row_in ? gy : 0) * pitch;
@@ -307,7 +307,7 @@ __device__ __forceinline__ int block_max(int v)
 // them with wave_inclusive_sum.  Returns, to every thread of the work-group,
 // whether every value of the window lies strictly within +-qrange (this is
 // also the barrier after staging).
-template <int TW, int TH>
+template <int TW, int TH, int NT = kThreads>
 __device__ __forceinline__ bool stage_prefix(uint32_t *__restrict__ q, int lp, const float *__restrict__ x, int W,
                                              int H, int pitch, int x0, int y0, int R, float qscale, float qrange)
 {
@@ -318,7 +318,7 @@ __device__ __forceinline__ bool stage_prefix(uint32_t *__restrict__ q, int lp, c
     const int c = 2 * lane, gc = x0 - R + c;
     const bool in0 = c < fw && gc >= 0 && gc < W, in1 = c + 1 < fw && gc + 1 >= 0 && gc + 1 < W;
     bool fits = true;
-    for (int r = tid >> 6; r < fh; r += kThreads / 64) {
+    for (int r = tid >> 6; r < fh; r += NT / 64) {
         const int gy = y0 - R + r;
         const bool row_in = gy >= 0 && gy < H;
         const auto xrow = gx + (size_t)(row_in ? gy : 0) * pitch;
@@ -332,7 +332,7 @@ __device__ __forceinline__ bool stage_prefix(uint32_t *__restrict__ q, int lp, c
         if (c + 1 < fw) row[2 + c] = incl;
         if (lane == 0) row[0] = 0u;
     }
-    return block_all(fits);
+    return block_vote<NT, true>(fits);
 }
 
 // rs[p] += fixed-point sum over the runs kr of the tile, for the P pixels
@@ -420,7 +420,7 @@ __device__ __forceinline__ void quantized_runs(const float *__restrict__ lane, i
 
 // stage_tile, also returning to every thread whether every staged value lies
 // strictly within +-qrange (this is also the barrier after staging).
-template <int TW, int TH>
+template <int TW, int TH, int NT = kThreads>
 __device__ __forceinline__ bool stage_tile_fits(float *__restrict__ tile, int lp, const float *__restrict__ x, int W,
                                                 int H, int pitch, int x0, int y0, int R, float qrange)
 {
@@ -428,7 +428,7 @@ __device__ __forceinline__ bool stage_tile_fits(float *__restrict__ tile, int lp
     const int fw = TW + 2 * R, fh = TH + 2 * R;
     const auto gx = as_global(x);
     bool fits = true;
-    for (int r = tid >> 6; r < fh; r += kThreads / 64) {
+    for (int r = tid >> 6; r < fh; r += NT / 64) {
         const int gy = y0 - R + r;
         const bool row_in = gy >= 0 && gy < H;
         const auto xrow = gx + (size_t)(row_in ? gy : 0) * pitch;
@@ -439,7 +439,7 @@ __device__ __forceinline__ bool stage_tile_fits(float *__restrict__ tile, int lp
             tile[r * lp + c] = v;
         }
     }
-    return block_all(fits);
+    return block_vote<NT, true>(fits);
 }
 
 }  // namespace lpa
