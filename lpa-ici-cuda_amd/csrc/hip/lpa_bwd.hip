@@ -31,8 +31,8 @@ namespace {
 constexpr int kMaxScales = 16;
 
 // Pass-1 tile: 64 x 32 output pixels, 8 adjacent pixels per lane, 256 lanes.
-constexpr int GW = 64, GH = 32, GP = 8, GLX = GW / GP;
-static_assert(GLX * GH == kThreads, "pass-1 tile must use 256 lanes");
+constexpr int GW = kBwdTileW, GH = kBwdTileH, GP = 8, GLX = GW / GP, GNT = GLX * GH;
+static_assert(GNT % 64 == 0 && GNT <= 1024, "pass-1 lanes: whole waves, at most 1024");
 
 // The float tile (pitch lds_pitch(GW, R, kBwdPitchResidue)) and the prefix
 // table (one more column, pitch lds_pitch(GW + 1, R)) share the start of the
@@ -254,7 +254,8 @@ constexpr int kPieceCost = 100, kLanePixelCost = 12, kMaxLaneSlices = 4;
 // Each work-group owns one row of `partial` ([grid][NW + K]); its waves add
 // into it with float atomics that return nothing.
 template <bool RING>
-__global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
+__global__ __launch_bounds__(GNT) __attribute__((amdgpu_waves_per_eu(GNT > kThreads ? 4 : 1)))
+void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                                                             const BwdDesc *__restrict__ imgs,
                                                             const uint2 *__restrict__ tiles, int num_tiles,
                                                             int *__restrict__ counter, int R,
@@ -281,16 +282,16 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
         const BwdDesc bd = load_const_struct(imgs + te.x);
         const ImgDesc &img = bd.img;
         const int x0 = (int)(te.y & 0xffffu) * GW, y0 = (int)(te.y >> 16) * GH;
-        for (int i = tid; i < 3 * K; i += kThreads) tab[i] = as_const(img.tab)[i];
+        for (int i = tid; i < 3 * K; i += GNT) tab[i] = as_const(img.tab)[i];
         // fixed: the forward's fixed-point ring sums apply to this window
         // (every value within the range); the vote is also the barrier.
         bool fixed = false;
         if constexpr (RING)
-            fixed = stage_prefix<GW, GH>(qt, lpq, img.x, img.W, img.H, img.pitch, x0, y0, R, rings.qscale,
+            fixed = stage_prefix<GW, GH, GNT>(qt, lpq, img.x, img.W, img.H, img.pitch, x0, y0, R, rings.qscale,
                                          rings.qrange);
         if (!fixed) {
             __syncthreads();  // every lane has voted before the window is restaged
-            stage_tile<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
+            stage_tile<GW, GH, GNT>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
             __syncthreads();
         }
 
@@ -377,7 +378,7 @@ __global__ __launch_bounds__(kThreads) void bwd_grad_kernel(DevKset ks, DevRings
             }
             if (fixed) {
                 __syncthreads();  // the prefix table is done with
-                stage_tile<GW, GH>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
+                stage_tile<GW, GH, GNT>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
                 __syncthreads();
             }
         }
@@ -630,13 +631,27 @@ size_t dx_lds_bytes(int R, int K)
     return sizeof(float) * (size_t)lds_floats(DXW, DXH, R);
 }
 
+// The first pass's LDS can pass the default limit of a launch: raise it.
+hipError_t allow_grad_lds(int R, int K)
+{
+    const size_t bytes = grad_lds_bytes(R, K);
+    if (bytes <= 65536) return hipSuccess;
+    for (const void *f : {reinterpret_cast<const void *>(&bwd_grad_kernel<true>),
+                          reinterpret_cast<const void *>(&bwd_grad_kernel<false>)}) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 }  // namespace
 
 void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, int *dx)
 {
     int b = 0;
     (void)num_scales;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_grad_kernel<false>, kThreads,
+    if (allow_grad_lds(radius, num_kernels) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_grad_kernel<true>, GNT,
                                                      grad_lds_bytes(radius, num_kernels)) != hipSuccess)
         b = 1;
     *prep = b > 0 ? b : 1;
@@ -654,16 +669,19 @@ hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAd
 {
     const int K = ks.D * ks.S, NW = adj.num_weights;
     if (ks.S > kMaxScales) return hipErrorInvalidValue;
-    if (grad_lds_bytes(radius, K) > 65536 || dx_lds_bytes(radius, K) > 65536) return hipErrorInvalidValue;
+    if (grad_lds_bytes(radius, K) > (size_t)kMaxLdsBytes || dx_lds_bytes(radius, K) > 65536)
+        return hipErrorInvalidValue;
+    const hipError_t ea = allow_grad_lds(radius, K);
+    if (ea != hipSuccess) return ea;
     const int rows = grid_prep;
     const size_t row_floats = (size_t)rows * (NW + K);
     hipLaunchKernelGGL(reset_counter2, dim3(1), dim3(1), 0, stream, d_counter);
     hipLaunchKernelGGL(fill_kernel, dim3(256), dim3(kThreads), 0, stream, partial, row_floats, 0.0f);
     if (rings.runs)
-        hipLaunchKernelGGL(bwd_grad_kernel<true>, dim3(grid_prep), dim3(kThreads), grad_lds_bytes(radius, K), stream,
+        hipLaunchKernelGGL(bwd_grad_kernel<true>, dim3(grid_prep), dim3(GNT), grad_lds_bytes(radius, K), stream,
                            ks, rings, adj, d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
     else
-        hipLaunchKernelGGL(bwd_grad_kernel<false>, dim3(grid_prep), dim3(kThreads), grad_lds_bytes(radius, K), stream,
+        hipLaunchKernelGGL(bwd_grad_kernel<false>, dim3(grid_prep), dim3(GNT), grad_lds_bytes(radius, K), stream,
                            ks, rings, adj, d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
     if (rings.runs)
         hipLaunchKernelGGL(bwd_dx_kernel<true>, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, rings,

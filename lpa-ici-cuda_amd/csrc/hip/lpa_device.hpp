@@ -154,7 +154,7 @@ hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &ri
 //   partial: [grid_prep][num_weights + K] floats (+1+2(num_weights+K) for the column sums in double), one row
 //            per work-group of the first pass, accumulated by its waves
 //   dkernel: += reduction of the rows plus the variance term (may be null)
-constexpr int kBwdTileW = 64, kBwdTileH = 32, kDxTile = 64;
+constexpr int kBwdTileW = 64, kBwdTileH = 64, kDxTile = 64;
 // Row pitch residue (mod 32 banks) of the first pass's float tile: with rows
 // 9 banks apart, the 64 taps a wave reads together when the taps of a kernel
 // are on the lanes spread over the banks for every direction of the shipped
