@@ -352,22 +352,38 @@ def main():
                      "executed_tops": round(main_leg["executed_tops"], 3),
                      "executed_frac_of_lane_ops": round(main_leg["executed_frac_of_lane_ops"], 4)},
     }
+    headline_config = ks.tile_config()
+
+    def tune_for(kset, flags, label):
+        """Autotune the tile configuration for this walk (unless one was given)."""
+        if args.tile_config >= 0:
+            kset.set_tile_config(args.tile_config)
+            return
+        from lpa_ici.autotune import autotune
+        best, times = autotune(kset, [torch.from_numpy(x_host).cuda()], flags=lpa_ici.FWD_PLAIN | flags,
+                               stream=stream, cache=None)
+        log(f"[autotune] {label}: best config {best}")
+
     if main_leg["walk"] == "fixed-point ring sums":
         for key, flags in (("float_ring_walk", lpa_ici.FWD_RING_FLOAT), ("weighted_walk", lpa_ici.FWD_WEIGHTED)):
+            tune_for(ks, flags, key)
             w = plain_leg(ks, x_host, args.steps, args.warmup, stream, f"plain symmetric {key}", flags=flags)
             rec[key] = {"mpx_s": round(w["mpx_s"], 2), "ms": round(w["ms"], 4), "tflops": round(w["tflops"], 3),
                         "frac": round(w["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4),
                         "executed_op": w["executed_op"], "executed_ops_per_px": round(w["executed_ops_per_px"], 1),
-                        "executed_frac_of_lane_ops": round(w["executed_frac_of_lane_ops"], 4)}
+                        "executed_frac_of_lane_ops": round(w["executed_frac_of_lane_ops"], 4),
+                        "tile_config": ks.tile_config()}
+        ks.set_tile_config(headline_config)
     if not args.skip_cpu:
         rec["cpu_baseline"] = cpu_baseline(x_host)
         log(f"[cpu] {rec['cpu_baseline']}")
     if not args.skip_gaussian:
         ksg = lpa_ici.load_kernels(lpa_ici.GAUSSIAN_M20_KERNELS)
-        ksg.set_tile_config(ks.tile_config())
+        tune_for(ksg, 0, "m20 gaussian")
         g = plain_leg(ksg, x_host, args.steps, args.warmup, stream, "plain m20 gaussian")
         rec["gaussian_m20"] = {"mpx_s": round(g["mpx_s"], 2), "ms": round(g["ms"], 4),
-                               "tflops": round(g["tflops"], 3), "taps_per_px": round(g["taps_per_px"], 1)}
+                               "tflops": round(g["tflops"], 3), "taps_per_px": round(g["taps_per_px"], 1),
+                               "tile_config": ksg.tile_config()}
     del main_leg
     if not args.skip_batched:
         b = batched_leg(ks, max(3, args.steps // 4), 1, stream)

@@ -155,7 +155,9 @@ typedef struct lpa_tile_config {
 int lpa_num_tile_configs(void);
 lpa_status lpa_tile_config_get(int index, lpa_tile_config* cfg);
 /* Make `index` the configuration used by later launches of this kernel set
- * (the autotuner's output); -1 restores the built-in choice. */
+ * (the autotuner's output); -1 restores the built-in choice: configuration
+ * 9 for the fixed-point ring walk, 7 for the other walks.  The getter
+ * returns the configuration of the set's default walk. */
 lpa_status lpa_kset_set_tile_config(lpa_kset_t ks, int index);
 int lpa_kset_get_tile_config(lpa_kset_t ks);
 

@@ -83,9 +83,11 @@ struct lpa_kset {
 
 namespace {
 
-// Tile configuration used until one is set (lpa_kset_set_tile_config): the
-// fastest on a 4096^2 image of the default set (1024 lanes, 64 x 64 tiles).
-constexpr int kDefaultShape = 9;
+// Tile configurations used until one is set (lpa_kset_set_tile_config), the
+// fastest on a 4096^2 image of the default set: 9 (1024 lanes within 64
+// registers) for the fixed-point ring walk; 7 (512 lanes within 128
+// registers) for the walks with more live values per pixel, which spill at 9.
+constexpr int kDefaultShapeFixedPoint = 9, kDefaultShape = 7;
 
 lpa_status hip_fail(hipError_t e, const char *what)
 {
@@ -99,7 +101,13 @@ lpa_status hip_fail(hipError_t e, const char *what)
         if (e_ != hipSuccess) return hip_fail(e_, #expr);                                                          \
     } while (0)
 
-int shape_of(const lpa_kset *ks) { return ks->shape >= 0 ? ks->shape : kDefaultShape; }
+lpa::WalkMode walk_mode(const lpa_kset *ks, unsigned flags);
+
+int shape_of(const lpa_kset *ks, unsigned flags)
+{
+    if (ks->shape >= 0) return ks->shape;
+    return walk_mode(ks, flags) == lpa::kWalkRing ? kDefaultShapeFixedPoint : kDefaultShape;
+}
 
 // Row-run packing: for every kernel row with a nonzero, one run from the
 // first to the last nonzero (interior zeros kept), offsets from the centre
@@ -372,7 +380,7 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
 {
     p->ks = ks;
     p->flags = flags;
-    p->shape = shape_of(ks);
+    p->shape = shape_of(ks, flags);
     p->n = n;
     const int K = ks->K;
     std::vector<uint2> tiles;
@@ -552,7 +560,7 @@ lpa_status lpa_kset_set_tile_config(lpa_kset_t ks, int index)
     return LPA_OK;
 }
 
-int lpa_kset_get_tile_config(lpa_kset_t ks) { return ks ? shape_of(ks) : -1; }
+int lpa_kset_get_tile_config(lpa_kset_t ks) { return ks ? shape_of(ks, 0) : -1; }
 
 lpa_status lpa_ici_forward(lpa_kset_t ks, const lpa_image *imgs, int n, unsigned flags, void *stream)
 {

@@ -4,8 +4,8 @@ For a kernel set and a workload (device images), every compiled tile
 configuration — tile shape, pixels per lane, inner-loop body — is timed with
 HIP events on the caller's stream, and the fastest is made the kernel set's
 configuration.  Results are cached in a JSON file keyed by the kernel set's
-SHA-256, the device name and the workload form (plain or batched), so a
-later process reuses the choice without timing again.
+SHA-256, the device name, the workload form (plain or batched) and the walk
+flags, so a later process reuses the choice without timing again.
 """
 import hashlib
 import json
@@ -74,7 +74,8 @@ def autotune(ks, xs, flags=FWD_PLAIN, sigmas=20.0 / 255.0, gammas=2.0, cache=DEF
     stream = stream or torch.cuda.current_stream()
     form = "plain" if flags & FWD_PLAIN else "batched"
     npx = sum(x.numel() for x in xs)
-    key = f"{kset_key(ks)}|{torch.cuda.get_device_name(0)}|{form}|{len(xs)}x{npx}"
+    walk = flags & ~FWD_PLAIN
+    key = f"{kset_key(ks)}|{torch.cuda.get_device_name(0)}|{form}|walk{walk}|{len(xs)}x{npx}"
     table = _load(cache) if cache else {}
     if not force and key in table:
         ks.set_tile_config(int(table[key]["best"]))
