@@ -168,10 +168,20 @@ def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
         ms_f = timed(lambda: fwd.run(stream), steps, warmup, stream)
         ms_fb = timed(step, steps, warmup, stream)
     npx = n * side * side
-    log(f"[fwd+bwd] {n}x{side}^2: fwd {ms_f:.2f} ms, fwd+bwd {ms_fb:.2f} ms, {npx / ms_fb / 1e3:.1f} Mpx/s")
+    # Algorithmic adjoint work: per selected kernel tap of every pixel and
+    # direction, 2 FLOP to recompute yhat, 2 for the dL/dx gather and 4 for
+    # dL/dkernel (the reference's taps, whatever the walks execute).
+    nnz = torch.from_numpy(ks.runs()["nnz"].astype(np.int64).reshape(ks.D, ks.S)).cuda()
+    dsel = torch.arange(ks.D, device="cuda")[:, None, None]
+    sel_taps = int(sum(nnz[dsel, m.long()].sum().item() for m in maps))
+    ms_b = ms_fb - ms_f
+    bwd_tflops = 8.0 * sel_taps / (ms_b * 1e-3) / 1e12
+    log(f"[fwd+bwd] {n}x{side}^2: fwd {ms_f:.2f} ms, fwd+bwd {ms_fb:.2f} ms, {npx / ms_fb / 1e3:.1f} Mpx/s; "
+        f"adjoint {ms_b:.2f} ms, {sel_taps / npx:.0f} selected taps/px, {bwd_tflops:.1f} TFLOP/s algorithmic")
     fwd.close()
     return {"images": n, "side": side, "fwd_ms": ms_f, "fwd_bwd_ms": ms_fb, "fwd_mpx_s": npx / ms_f / 1e3,
-            "fwd_bwd_mpx_s": npx / ms_fb / 1e3}
+            "fwd_bwd_mpx_s": npx / ms_fb / 1e3, "bwd_ms": ms_b, "selected_taps_per_px": sel_taps / npx,
+            "bwd_tflops_algorithmic": bwd_tflops}
 
 
 def measured_traffic(walk):
