@@ -30,7 +30,8 @@ namespace {
 
 constexpr int kMaxScales = 16;
 
-// Pass-1 tile: 64 x 32 output pixels, 8 adjacent pixels per lane, 256 lanes.
+// Pass-1 tile: 64 x 64 output pixels, 8 adjacent pixels per lane, 512 lanes
+// held to 128 registers (two work-groups per CU).
 constexpr int GW = kBwdTileW, GH = kBwdTileH, GP = 8, GLX = GW / GP, GNT = GLX * GH;
 static_assert(GNT % 64 == 0 && GNT <= 1024, "pass-1 lanes: whole waves, at most 1024");
 

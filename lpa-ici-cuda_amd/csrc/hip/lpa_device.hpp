@@ -148,7 +148,7 @@ hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &ri
                             const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
                             hipStream_t stream);
 
-// Adjoint (lpa_bwd.hip).  The first pass walks 64 x 32 tiles (d_tiles), the
+// Adjoint (lpa_bwd.hip).  The first pass walks 64 x 64 tiles (d_tiles), the
 // dx pass 64 x 64 tiles (dx_tiles); both queues are laid out as for the
 // batched forward ({image, (tile_y << 16) | tile_x}).
 //   partial: [grid_prep][num_weights + K] floats (+1+2(num_weights+K) for the column sums in double), one row
