@@ -452,6 +452,59 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
     }
 }
 
+// Unweighted run sums for the P pixels starting at `src` (the masked map of
+// the dx pass): acc[p] += sum_{j < L} src[p + j].  From L = 4 on, the first
+// pixel's sum slides to the next pixels ((s + src[p + L - 1]) - src[p - 1]):
+// L + 3P - 3 operations instead of L P.  The dx pass has no bit-exact
+// counterpart to match (its bar is the FP64 adjoint); the two extra
+// roundings per slide are far below it.
+template <int L, int P>
+__device__ __forceinline__ void run_sums_sliding(const float *__restrict__ src, float (&acc)[P])
+{
+    float v[L + P - 1];
+#pragma unroll
+    for (int q = 0; q < L + P - 1; ++q) v[q] = src[q];
+    if constexpr (L < 4) {
+#pragma unroll
+        for (int j = 0; j < L; ++j)
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[p] += v[j + p];
+    } else {
+        float sum = v[0];
+#pragma unroll
+        for (int j = 1; j < L; ++j) sum += v[j];
+        acc[0] += sum;
+#pragma unroll
+        for (int p = 1; p < P; ++p) {
+            sum = (sum + v[p + L - 1]) - v[p - 1];
+            acc[p] += sum;
+        }
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void ring_sums_sliding(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
+                                                  int2 kr, float (&acc)[P])
+{
+    for (int r = 0; r < kr.y; ++r) {
+        const int4 run = load_const(runs + kr.x + r);
+        const float *__restrict__ src = lane + __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
+        switch (run.z) {
+#define LPA_SLIDE_CASE(L)                                                                                          \
+    case L: run_sums_sliding<L, P>(src, acc); break;
+            LPA_SLIDE_CASE(1) LPA_SLIDE_CASE(2) LPA_SLIDE_CASE(3) LPA_SLIDE_CASE(4) LPA_SLIDE_CASE(5)
+            LPA_SLIDE_CASE(6) LPA_SLIDE_CASE(7) LPA_SLIDE_CASE(8) LPA_SLIDE_CASE(9) LPA_SLIDE_CASE(10)
+            LPA_SLIDE_CASE(11) LPA_SLIDE_CASE(12) LPA_SLIDE_CASE(13) LPA_SLIDE_CASE(14) LPA_SLIDE_CASE(15)
+            LPA_SLIDE_CASE(16) LPA_SLIDE_CASE(17) LPA_SLIDE_CASE(18) LPA_SLIDE_CASE(19) LPA_SLIDE_CASE(20)
+            LPA_SLIDE_CASE(21) LPA_SLIDE_CASE(22) LPA_SLIDE_CASE(23) LPA_SLIDE_CASE(24) LPA_SLIDE_CASE(25)
+            LPA_SLIDE_CASE(26) LPA_SLIDE_CASE(27) LPA_SLIDE_CASE(28) LPA_SLIDE_CASE(29) LPA_SLIDE_CASE(30)
+            LPA_SLIDE_CASE(31) LPA_SLIDE_CASE(32)
+#undef LPA_SLIDE_CASE
+        default: break;
+        }
+    }
+}
+
 // Pass 2 (dx), on 64 x 64 tiles with 1024 lanes (4 px per lane).  Each lane
 // keeps q and the scale index of 16 tile+halo elements in registers (rows
 // tid/64 + 16i, columns tid%64 + 64j); the LDS holds only the masked map and
@@ -552,7 +605,7 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings,
                                 mt[r * lp + c] = sv[i][j] == s ? qv[i][j] * cw : 0.0f;
                         }
                     __syncthreads();
-                    correlate_runs<DXP, false, 1, false>(lane_m, lp, rings.flip, nullptr, load_const(rings.kern + k), dx);
+                    ring_sums_sliding<DXP>(lane_m, lp, rings.flip, load_const(rings.kern + k), dx);
                 }
             } else {
                 for (int s = 0; s < S; ++s) {
