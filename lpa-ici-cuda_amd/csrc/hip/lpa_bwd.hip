@@ -169,6 +169,25 @@ __device__ __forceinline__ void grad_piece(const float *__restrict__ src, const 
     reduce_scatter_add<L>(part, row, lane);
 }
 
+// The lane's GP scale indices in one direction's map (255 for pixels outside
+// the image): one 8-byte load where the lane's pixels are whole and aligned
+// (`whole`: an 8-aligned map of an image whose width is a multiple of 8),
+// else byte by byte.
+__device__ __forceinline__ void load_scales(const uint8_t *__restrict__ m, bool whole, const bool (&inside)[GP],
+                                            int (&sd)[GP])
+{
+    if (whole) {
+        const unsigned long long v =
+            *reinterpret_cast<const __attribute__((address_space(1))) unsigned long long *>(as_global(m));
+#pragma unroll
+        for (int p = 0; p < GP; ++p) sd[p] = (int)((v >> (8 * p)) & 255u);
+    } else {
+        const auto g = as_global(m);
+#pragma unroll
+        for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? g[p] : 255;
+    }
+}
+
 // A partials of one kernel for the few pixels of a wave that selected it,
 // with the kernel's taps on the lanes (tap t = 64 i + lane, IT = number of
 // 64-tap slices): every pixel p with coefficient c_p adds c_p x[p + t] to its
@@ -302,19 +321,28 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
         const float *lane_x = xt + (ty + R) * lp + tx * GP + R;
         const size_t plane = (size_t)img.W * img.H;
         const size_t pix = (size_t)py * img.W + px0;
-        const auto gmap = as_global(img.map) + pix;
         const auto gg = as_global(bd.g) + (size_t)py * img.pitch + px0;
         const auto gy = as_global(img.y) + (size_t)py * img.pitch + px0;
         const auto gq = as_global(bd.q) + pix;
         bool inside[GP];
-        float q[GP], yv[GP];
+        float q[GP], yv[GP], wt[GP];
 #pragma unroll
         for (int p = 0; p < GP; ++p) {
             inside[p] = py < img.H && px0 + p < img.W;
-            float wt = 0.0f;
-            if (inside[p])
-                for (int d = 0; d < D; ++d) wt += inv_t[d * S + min((int)gmap[(size_t)d * plane + p], S - 1)];
-            q[p] = inside[p] ? gg[p] / wt : 0.0f;
+            wt[p] = 0.0f;
+        }
+        const bool whole = py < img.H && px0 < img.W && img.W % GP == 0 &&
+                           ((uintptr_t)img.map & (GP - 1)) == 0;
+        for (int d = 0; d < D; ++d) {
+            int sd[GP];
+            load_scales(img.map + (size_t)d * plane + pix, whole, inside, sd);
+#pragma unroll
+            for (int p = 0; p < GP; ++p)
+                if (inside[p]) wt[p] += inv_t[d * S + min(sd[p], S - 1)];
+        }
+#pragma unroll
+        for (int p = 0; p < GP; ++p) {
+            q[p] = inside[p] ? gg[p] / wt[p] : 0.0f;
             yv[p] = inside[p] ? gy[p] : 0.0f;
             if (inside[p]) gq[p] = q[p];
         }
@@ -325,9 +353,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
             const uint32_t *lane_q = qt + (ty + R) * lpq + tx * GP + R;
             for (int d = 0; d < D; ++d) {
                 int sd[GP];
-                const auto gmd = gmap + (size_t)d * plane;
-#pragma unroll
-                for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? gmd[p] : 255;
+                load_scales(img.map + (size_t)d * plane + pix, whole, inside, sd);
                 // The forward's ring walk: running sum over rings 0..s up to
                 // the deepest selected scale; yhat = (w_s 2^-F) * (fixed-point
                 // sum), or w_s * (float sum) for a window outside the range.
@@ -386,9 +412,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
 
         for (int d = 0; d < D; ++d) {
             int sd[GP];
-            const auto gmd = gmap + (size_t)d * plane;
-#pragma unroll
-            for (int p = 0; p < GP; ++p) sd[p] = inside[p] ? gmd[p] : 255;
+            load_scales(img.map + (size_t)d * plane + pix, whole, inside, sd);
             for (int s = 0; s < S; ++s) {
                 bool want = false;
 #pragma unroll

@@ -105,6 +105,15 @@ def test_adjoint_weighted_kernel_set():
     assert e_dx < 1e-4 and e_dk < 1e-4
 
 
+@pytest.mark.parametrize("w,h", [(70, 45), (131, 9)])
+def test_adjoint_ragged_width(ks, w, h):
+    """Widths that are not a multiple of 8: the first pass reads the scale
+    maps byte by byte, and the last tile of every row is partial."""
+    e_dx, e_dk = adjoint_error(ks, noisy_synth(w, h, seed=w + h))
+    print(f"{w}x{h} vs FP64: dx rel {e_dx:.2e}, dk rel {e_dk:.2e}")
+    assert e_dx < 1e-4 and e_dk < 1e-4
+
+
 def test_adjoint_fixed_point_fallback(ks):
     """Values outside the fixed-point range in one corner: the windows within
     reach of them recompute yhat with float ring sums, as the forward does,
