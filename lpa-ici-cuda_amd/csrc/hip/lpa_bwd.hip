@@ -34,6 +34,13 @@ constexpr int kMaxScales = 16;
 // held to 128 registers (two work-groups per CU).
 constexpr int GW = kBwdTileW, GH = kBwdTileH, GP = 8, GLX = GW / GP, GNT = GLX * GH;
 static_assert(GNT % 64 == 0 && GNT <= 1024, "pass-1 lanes: whole waves, at most 1024");
+// A wave covers 8 rows of the tile.  Each half-wave (whose 32 lanes the LDS
+// serves together) takes a 32-pixel-wide half of them: 4 lanes per row, 8 px
+// apart, on rows 9 (float tile) or 1 (prefix table) banks apart, so the 32
+// lanes of a read hit 32 distinct banks.
+static_assert(GLX == 8, "the lane layout assumes 8 lanes per tile row");
+__device__ __forceinline__ int grad_lane_col(int lane) { return (lane >> 5) * 4 + (lane & 3); }
+__device__ __forceinline__ int grad_lane_row(int lane) { return (lane & 31) >> 2; }
 
 // The float tile (pitch lds_pitch(GW, R, kBwdPitchResidue)) and the prefix
 // table (one more column, pitch lds_pitch(GW + 1, R)) share the start of the
@@ -209,10 +216,10 @@ __device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int 
     float acc[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) acc[i] = 0.0f;
-    // lane L holds pixels (L % GLX) * GP + p of row L / GLX of the wave
+    // lane L holds pixels grad_lane_col(L) * GP + p of row grad_lane_row(L) of the wave
     auto pixel = [&](int L, int p, float &c) {
         c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m[p]), L));
-        return base0 + (L / GLX) * lp + (L % GLX) * GP + p;
+        return base0 + grad_lane_row(L) * lp + grad_lane_col(L) * GP + p;
     };
 #pragma unroll
     for (int p = 0; p < GP; ++p) {
@@ -316,7 +323,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
         }
 
         const float *inv_t = tab + 2 * K;
-        const int tx = tid % GLX, ty = tid / GLX;
+        const int tx = grad_lane_col(lane), ty = wave * (64 / GLX) + grad_lane_row(lane);
         const int px0 = x0 + tx * GP, py = y0 + ty;
         const float *lane_x = xt + (ty + R) * lp + tx * GP + R;
         const size_t plane = (size_t)img.W * img.H;
