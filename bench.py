@@ -13,10 +13,12 @@ carries
   batched_uneven  256 images with sides U{128..2048}, per-image sigma/gamma:
                 plain (one launch per image) vs one persistent launch vs its
                 hipGraph replay
-  fwd_bwd       64 x 512^2 forward + adjoint (dL/dx, dL/dkernel)
+  fwd_bwd       64 x 512^2 forward + adjoint (dL/dx, dL/dkernel); the forward
+                keeps the uint8 scale maps and each direction's selected
+                estimate, which the adjoint reads
   config5       32 images with sides U{1024..8192} (about 680 Mpx), fwd+bwd
-                with the estimates recomputed from the uint8 scale maps, and
-                the peak device memory in use against the 288 GB of HBM
+                as above, and the peak device memory in use against the
+                288 GB of HBM
 Flags --skip-* drop the side legs.
 """
 import argparse
@@ -144,14 +146,17 @@ def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
     xs = [torch.from_numpy(make_image(side, side, 2000 + i, SIGMA, SEED + i)).cuda() for i in range(n)]
     ys = [torch.empty_like(x) for x in xs]
     maps = [torch.empty((ks.D, side, side), dtype=torch.uint8, device="cuda") for _ in xs]
+    # the forward keeps each direction's selected estimate for the adjoint
+    ests = [torch.empty((ks.D, side, side), dtype=torch.float32, device="cuda") for _ in xs]
     g = torch.Generator(device="cuda").manual_seed(5)
     dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
     dxs = [torch.empty_like(x) for x in xs]
     nw = ks.info()["num_weights"]
     dk = torch.zeros(nw, dtype=torch.float32, device="cuda")
-    fwd = lpa_ici.Plan(ks, xs, ys, maps, SIGMA, GAMMA, flags=0)
+    fwd = lpa_ici.Plan(ks, xs, ys, maps, SIGMA, GAMMA, flags=0, estimates=ests)
     import ctypes
-    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, SIGMA, GAMMA) for x, y, m in zip(xs, ys, maps)])
+    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, SIGMA, GAMMA, None, e)
+                                   for x, y, m, e in zip(xs, ys, maps, ests)])
     dy_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dys])
     dx_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dxs])
     sp = ctypes.c_void_p(stream.cuda_stream)
@@ -199,8 +204,8 @@ def measured_traffic(walk):
 
 def config5_leg(ks, steps, warmup, stream, n=32, seed=5):
     """32 images with sides U{1024..8192}, per-image sigma/gamma as the batched
-    leg: forward (scale maps kept as uint8) + adjoint that recomputes the
-    estimates from the maps; peak HBM use reported."""
+    leg: forward (scale maps kept as uint8, per-direction estimates as
+    float32) + adjoint that reads them; peak HBM use reported."""
     import ctypes
     rng = np.random.default_rng(seed)
     sides = rng.integers(1024, 8193, size=(n, 2))
@@ -216,13 +221,14 @@ def config5_leg(ks, steps, warmup, stream, n=32, seed=5):
     log(f"[config5] generated {n} images, {npx / 1e6:.1f} Mpx in {time.time() - t0:.1f} s")
     ys = [torch.empty_like(x) for x in xs]
     maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda") for x in xs]
+    ests = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.float32, device="cuda") for x in xs]
     g = torch.Generator(device="cuda").manual_seed(7)
     dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
     dxs = [torch.empty_like(x) for x in xs]
     dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
-    fwd = lpa_ici.Plan(ks, xs, ys, maps, sig, gam, flags=0)
-    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, s_, g_) for x, y, m, s_, g_ in
-                                   zip(xs, ys, maps, sig, gam)])
+    fwd = lpa_ici.Plan(ks, xs, ys, maps, sig, gam, flags=0, estimates=ests)
+    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, s_, g_, None, e) for x, y, m, s_, g_, e in
+                                   zip(xs, ys, maps, sig, gam, ests)])
     dy_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dys])
     dx_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dxs])
     sp = ctypes.c_void_p(stream.cuda_stream)

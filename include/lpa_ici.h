@@ -106,6 +106,12 @@ typedef struct lpa_image {
     const float* variances; /* host, may be NULL: D*S variances used instead of
                                sigma^2 * sum(w^2) (the reference's
                                host_kernel_variances argument) */
+    float* estimates;    /* device, may be NULL: [D][height][width] (dense, pitch =
+                            width), each direction's estimate at its selected
+                            scale.  lpa_ici_forward writes it (the values it
+                            aggregated, bit for bit); lpa_ici_backward reads it,
+                            for ring-form sets, instead of recomputing them.
+                            D*H*W floats of HBM traded for the recomputation. */
 } lpa_image;
 
 enum {

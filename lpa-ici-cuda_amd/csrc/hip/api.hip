@@ -414,6 +414,7 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
         d.x = m.x;
         d.y = m.y;
         d.map = m.scale_map;
+        d.est = m.estimates;
         d.tab = d_tab + (size_t)3 * K * i;
         d.W = m.width;
         d.H = m.height;
@@ -614,6 +615,7 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
         e.img.x = m.x;
         e.img.y = m.y;
         e.img.map = m.scale_map;
+        e.img.est = nullptr;
         e.img.tab = nullptr;
         e.img.W = m.width;
         e.img.H = m.height;
@@ -697,6 +699,7 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
         b.img.x = m.x;
         b.img.y = m.y;
         b.img.map = m.scale_map;
+        b.img.est = m.estimates;
         b.img.tab = (const float *)(base + off_tab) + (size_t)3 * K * i;
         b.img.W = m.width;
         b.img.H = m.height;

@@ -312,10 +312,14 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
         for (int i = tid; i < 3 * K; i += GNT) tab[i] = as_const(img.tab)[i];
         // fixed: the forward's fixed-point ring sums apply to this window
         // (every value within the range); the vote is also the barrier.
+        // With the forward's estimates at hand nothing is recomputed, and the
+        // float tile is staged at once.
+        const bool have_est = img.est != nullptr;
         bool fixed = false;
         if constexpr (RING)
-            fixed = stage_prefix<GW, GH, GNT>(qt, lpq, img.x, img.W, img.H, img.pitch, x0, y0, R, rings.qscale,
-                                         rings.qrange);
+            if (!have_est)
+                fixed = stage_prefix<GW, GH, GNT>(qt, lpq, img.x, img.W, img.H, img.pitch, x0, y0, R, rings.qscale,
+                                                  rings.qrange);
         if (!fixed) {
             __syncthreads();  // every lane has voted before the window is restaged
             stage_tile<GW, GH, GNT>(xt, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
@@ -361,37 +365,44 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
             for (int d = 0; d < D; ++d) {
                 int sd[GP];
                 load_scales(img.map + (size_t)d * plane + pix, whole, inside, sd);
-                // The forward's ring walk: running sum over rings 0..s up to
-                // the deepest selected scale; yhat = (w_s 2^-F) * (fixed-point
-                // sum), or w_s * (float sum) for a window outside the range.
-                float yh[GP], rs[GP];
-                uint32_t rq[GP];
+                float yh[GP];
+                if (have_est) {
+                    const auto ge = as_global(img.est) + (size_t)d * plane + pix;
 #pragma unroll
-                for (int p = 0; p < GP; ++p) {
-                    yh[p] = 0.0f;
-                    rs[p] = 0.0f;
-                    rq[p] = 0u;
-                }
-                for (int s = 0; s < S; ++s) {
-                    bool need = false;
+                    for (int p = 0; p < GP; ++p) yh[p] = inside[p] ? ge[p] : 0.0f;
+                } else {
+                    // The forward's ring walk: running sum over rings 0..s up to
+                    // the deepest selected scale; yhat = (w_s 2^-F) * (fixed-point
+                    // sum), or w_s * (float sum) for a window outside the range.
+                    float rs[GP];
+                    uint32_t rq[GP];
 #pragma unroll
-                    for (int p = 0; p < GP; ++p) need |= inside[p] && sd[p] >= s;
-                    if (!__any(need)) break;
-                    const int k = d * S + s;
-                    if (fixed) {
-                        if (need) prefix_runs<GP>(lane_q, lpq, rings.runs, load_const(rings.kern + k), rq);
-                        const float wq = as_const(rings.scale)[k] * rings.qinv;
+                    for (int p = 0; p < GP; ++p) {
+                        yh[p] = 0.0f;
+                        rs[p] = 0.0f;
+                        rq[p] = 0u;
+                    }
+                    for (int s = 0; s < S; ++s) {
+                        bool need = false;
 #pragma unroll
-                        for (int p = 0; p < GP; ++p)
-                            if (sd[p] == s) yh[p] = mul_rounded((float)(int)rq[p], wq);
-                    } else {
-                        if (need)
-                            correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr,
-                                                                 load_const(rings.kern + k), rs);
-                        const float w = as_const(rings.scale)[k];
+                        for (int p = 0; p < GP; ++p) need |= inside[p] && sd[p] >= s;
+                        if (!__any(need)) break;
+                        const int k = d * S + s;
+                        if (fixed) {
+                            if (need) prefix_runs<GP>(lane_q, lpq, rings.runs, load_const(rings.kern + k), rq);
+                            const float wq = as_const(rings.scale)[k] * rings.qinv;
 #pragma unroll
-                        for (int p = 0; p < GP; ++p)
-                            if (sd[p] == s) yh[p] = rs[p] * w;
+                            for (int p = 0; p < GP; ++p)
+                                if (sd[p] == s) yh[p] = mul_rounded((float)(int)rq[p], wq);
+                        } else {
+                            if (need)
+                                correlate_runs<GP, false, 1, false>(lane_x, lp, rings.runs, nullptr,
+                                                                     load_const(rings.kern + k), rs);
+                            const float w = as_const(rings.scale)[k];
+#pragma unroll
+                            for (int p = 0; p < GP; ++p)
+                                if (sd[p] == s) yh[p] = rs[p] * w;
+                        }
                     }
                 }
                 // E_k: sum over the pixels that selected s of c (yhat - y).

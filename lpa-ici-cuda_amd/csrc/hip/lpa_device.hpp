@@ -51,6 +51,7 @@ struct ImgDesc {
     const float *tab;
     int W, H, pitch;
     int tiles_x;
+    float *est;  // [D][H][W] selected estimates (forward output, adjoint input), may be null
 };
 
 // Compiled tile shapes: output tile tw x th pixels, px adjacent pixels per

@@ -29,8 +29,8 @@ namespace {
 template <int P, int MODE, int BODY, typename T>
 __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, const DevKset &ks, const DevRings &rings,
                                           const float *__restrict__ tab, const float *__restrict__ inv_lds,
-                                          float *__restrict__ y,
-                                          uint8_t *__restrict__ map, int W, int H, int pitch, int px0, int py)
+                                          float *__restrict__ y, uint8_t *__restrict__ map,
+                                          float *__restrict__ est, int W, int H, int pitch, int px0, int py)
 {
     constexpr bool EXACT = MODE == kWalkExact;
     const int S = ks.S, K = ks.D * ks.S;
@@ -123,6 +123,12 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
             for (int p = 0; p < P; ++p)
                 if (inside[p]) gmap[p] = (uint8_t)max(sel[p], 0);
         }
+        if (est) {
+            const auto gest = as_global(est) + (size_t)d * plane + pix;
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+                if (inside[p]) gest[p] = best[p];
+        }
     }
     const auto yrow = as_global(y) + (size_t)py * pitch + px0;
 #pragma unroll
@@ -140,8 +146,8 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
 template <int TW, int TH, int P, int MODE, int BODY, int PM, int NT, int SPLIT>
 __device__ __forceinline__ void process_tile(float *__restrict__ tile, const DevKset &ks, const DevRings &rings,
                                              const float *__restrict__ tab, const float *__restrict__ x,
-                                             float *__restrict__ y, uint8_t *__restrict__ map, int W, int H,
-                                             int pitch, int x0, int y0, int R)
+                                             float *__restrict__ y, uint8_t *__restrict__ map,
+                                             float *__restrict__ est, int W, int H, int pitch, int x0, int y0, int R)
 {
     constexpr int LX = TW / P;
     static_assert(LX * TH == NT, "tile shape must use every lane of the work-group");
@@ -167,20 +173,22 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
     if constexpr (MODE == kWalkRing) {
         uint32_t *q = (uint32_t *)tile;
         if (stage_prefix<TW, TH, NT>(q, lp, x, W, H, pitch, x0, y0, R, rings.qscale, rings.qrange)) {
-            walk_lane<P, kWalkRing, BODY>(q + lane_off, lp, ks, rings, tab, inv_lds, y, map, W, H, pitch, px0, py);
+            walk_lane<P, kWalkRing, BODY>(q + lane_off, lp, ks, rings, tab, inv_lds, y, map, est, W, H, pitch, px0,
+                                          py);
             return;
         }
         // A value of the window lies outside the fixed-point range (or is
         // not finite): float ring sums.  Every lane takes this branch.
         stage_tile<TW, TH, NT>(tile + 1, lp, x, W, H, pitch, x0, y0, R);
         __syncthreads();
-        walk_lane<P, kWalkRingFloat, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, inv_lds, y, map, W, H, pitch, px0,
-                                           py);
+        walk_lane<P, kWalkRingFloat, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, inv_lds, y, map, est, W, H, pitch,
+                                           px0, py);
     } else {
         // Pixels outside the image read as zero: the reference's zero padding.
         stage_tile<TW, TH, NT>(tile + 1, lp, x, W, H, pitch, x0, y0, R);
         __syncthreads();
-        walk_lane<P, MODE, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, inv_lds, y, map, W, H, pitch, px0, py);
+        walk_lane<P, MODE, BODY>(tile + 1 + lane_off, lp, ks, rings, tab, inv_lds, y, map, est, W, H, pitch, px0,
+                                 py);
     }
 }
 
@@ -311,7 +319,7 @@ void fwd_plain_kernel(DevKset ks, DevRings rings, ImgDesc img, int R, int num_ti
     if (t >= num_tiles) return;  // whole work-group, before any barrier
     const int tx = t % img.tiles_x, ty = t / img.tiles_x;
     process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm, sh.nt, sh.split>(
-        tile, ks, rings, img.tab, img.x, img.y, img.map, img.W, img.H, img.pitch, tx * sh.tw, ty * sh.th, R);
+        tile, ks, rings, img.tab, img.x, img.y, img.map, img.est, img.W, img.H, img.pitch, tx * sh.tw, ty * sh.th, R);
 }
 
 // Batched form: a persistent grid drains one tile queue that spans every
@@ -333,7 +341,8 @@ void fwd_batched_kernel(DevKset ks, DevRings rings, const ImgDesc *__restrict__ 
         const uint2 e = load_const(tiles + t);
         const ImgDesc img = load_const_struct(imgs + e.x);
         process_tile<sh.tw, sh.th, sh.px, MODE, sh.body, sh.pm, sh.nt, sh.split>(
-            tile, ks, rings, img.tab, img.x, img.y, img.map, img.W, img.H, img.pitch, (int)(e.y & 0xffffu) * sh.tw,
+            tile, ks, rings, img.tab, img.x, img.y, img.map, img.est, img.W, img.H, img.pitch,
+            (int)(e.y & 0xffffu) * sh.tw,
             (int)(e.y >> 16) * sh.th, R);
         __syncthreads();  // the LDS tile and `next` are reused
     }

@@ -246,9 +246,10 @@ def _stream_ptr(stream):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def image_struct(x, y, scale_map=None, sigma=20.0 / 255.0, gamma=2.0, variances=None):
+def image_struct(x, y, scale_map=None, sigma=20.0 / 255.0, gamma=2.0, variances=None, estimates=None):
     """An lpa_image over device tensors x, y (H x W float32, row stride =
-    pitch) and an optional uint8 [D, H, W] scale map."""
+    pitch), an optional uint8 [D, H, W] scale map and optional float32
+    [D, H, W] per-direction estimates."""
     torch = _torch()
     assert x.is_cuda and y.is_cuda and x.dtype == torch.float32 and y.dtype == torch.float32
     assert x.dim() == 2 and x.shape == y.shape and x.stride(1) == 1 and y.stride(1) == 1
@@ -256,6 +257,9 @@ def image_struct(x, y, scale_map=None, sigma=20.0 / 255.0, gamma=2.0, variances=
     if scale_map is not None:
         assert scale_map.dtype == torch.uint8 and scale_map.is_contiguous()
         assert scale_map.shape[1:] == x.shape
+    if estimates is not None:
+        assert estimates.dtype == torch.float32 and estimates.is_contiguous() and estimates.is_cuda
+        assert estimates.shape[1:] == x.shape
     img = LpaImage()
     img.x, img.y = x.data_ptr(), y.data_ptr()
     img.scale_map = scale_map.data_ptr() if scale_map is not None else None
@@ -263,15 +267,19 @@ def image_struct(x, y, scale_map=None, sigma=20.0 / 255.0, gamma=2.0, variances=
     img.pitch = int(x.stride(0))
     img.sigma, img.gamma = float(sigma), float(gamma)
     img.variances = variances.ctypes.data if variances is not None else None
+    img.estimates = estimates.data_ptr() if estimates is not None else None
     return img
 
 
-def denoise(ks, xs, sigmas=20.0 / 255.0, gammas=2.0, scale_maps=False, flags=0, stream=None, variances=None):
+def denoise(ks, xs, sigmas=20.0 / 255.0, gammas=2.0, scale_maps=False, flags=0, stream=None, variances=None,
+            keep_estimates=False):
     """Denoise one (H x W tensor) or several (list) device images.
 
-    Returns y (and the uint8 [D,H,W] scale maps if scale_maps=True), a list
-    when a list went in.  flags: FWD_PLAIN for one launch per image, FWD_EXACT
-    for separately rounded mul/add (the CPU path's bits)."""
+    Returns y (and the uint8 [D,H,W] scale maps if scale_maps=True, and the
+    float32 [D,H,W] per-direction estimates at the selected scales if
+    keep_estimates=True, in that order), lists when a list went in.  flags:
+    FWD_PLAIN for one launch per image, FWD_EXACT for separately rounded
+    mul/add (the CPU path's bits)."""
     torch = _torch()
     single = not isinstance(xs, (list, tuple))
     xs = [xs] if single else list(xs)
@@ -283,12 +291,15 @@ def denoise(ks, xs, sigmas=20.0 / 255.0, gammas=2.0, scale_maps=False, flags=0, 
     ys = [torch.empty_like(x) for x in xs]
     maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device=x.device) for x in xs] \
         if scale_maps else [None] * n
-    arr = (LpaImage * n)(*[image_struct(x, y, m, s, g, v) for x, y, m, s, g, v in zip(xs, ys, maps, sig, gam, var)])
+    ests = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.float32, device=x.device) for x in xs] \
+        if keep_estimates else [None] * n
+    arr = (LpaImage * n)(*[image_struct(x, y, m, s, g, v, e)
+                           for x, y, m, s, g, v, e in zip(xs, ys, maps, sig, gam, var, ests)])
     check(lib().lpa_ici_forward(ks.handle, arr, n, int(flags), _stream_ptr(stream)))
-    out = (ys, maps) if scale_maps else ys
+    out = [ys] + ([maps] if scale_maps else []) + ([ests] if keep_estimates else [])
     if single:
-        return (ys[0], maps[0]) if scale_maps else ys[0]
-    return out
+        out = [o[0] for o in out]
+    return out[0] if len(out) == 1 else tuple(out)
 
 
 def estimates(ks, xs, maps, flags=0, stream=None):
@@ -307,13 +318,15 @@ def estimates(ks, xs, maps, flags=0, stream=None):
     return outs[0] if single else outs
 
 
-def backward(ks, xs, ys, maps, dys, sigmas=20.0 / 255.0, gammas=2.0, dkernel=None, stream=None):
+def backward(ks, xs, ys, maps, dys, sigmas=20.0 / 255.0, gammas=2.0, dkernel=None, stream=None, estimates=None):
     """Adjoint of the forward with the scales frozen at `maps`.
 
     xs: the forward's inputs, ys: its outputs, maps: its uint8 scale maps,
-    dys: upstream gradients (all lists of device tensors).  Returns (dxs,
-    dkernel) where dkernel (float32, one entry per run weight of the kernel
-    set, see KernelSet.runs()) is accumulated into if given."""
+    dys: upstream gradients (all lists of device tensors); estimates: the
+    forward's per-direction estimates (keep_estimates=True), used instead of
+    recomputing them.  Returns (dxs, dkernel) where dkernel (float32, one
+    entry per run weight of the kernel set, see KernelSet.runs()) is
+    accumulated into if given."""
     torch = _torch()
     n = len(xs)
     sig = sigmas if isinstance(sigmas, (list, tuple)) else [sigmas] * n
@@ -323,7 +336,9 @@ def backward(ks, xs, ys, maps, dys, sigmas=20.0 / 255.0, gammas=2.0, dkernel=Non
         assert d.shape == y.shape and d.stride() == y.stride() and d.dtype == torch.float32
     if dkernel is None:
         dkernel = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device=xs[0].device)
-    arr = (LpaImage * n)(*[image_struct(x, y, m, s, g) for x, y, m, s, g in zip(xs, ys, maps, sig, gam)])
+    ests = list(estimates) if estimates is not None else [None] * n
+    arr = (LpaImage * n)(*[image_struct(x, y, m, s, g, None, e)
+                           for x, y, m, s, g, e in zip(xs, ys, maps, sig, gam, ests)])
     dy_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dys])
     dx_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dxs])
     check(lib().lpa_ici_backward(ks.handle, arr, dy_p, dx_p, ctypes.c_void_p(dkernel.data_ptr()), n,
@@ -337,13 +352,15 @@ class Plan:
     replayed from a hipGraph with FWD_GRAPH).  Keeps references to the
     tensors it was built over."""
 
-    def __init__(self, ks, xs, ys, maps=None, sigmas=20.0 / 255.0, gammas=2.0, flags=0):
+    def __init__(self, ks, xs, ys, maps=None, sigmas=20.0 / 255.0, gammas=2.0, flags=0, estimates=None):
         n = len(xs)
         maps = maps if maps is not None else [None] * n
+        ests = estimates if estimates is not None else [None] * n
         sig = sigmas if isinstance(sigmas, (list, tuple)) else [sigmas] * n
         gam = gammas if isinstance(gammas, (list, tuple)) else [gammas] * n
-        self._keep = (ks, list(xs), list(ys), list(maps))
-        arr = (LpaImage * n)(*[image_struct(x, y, m, s, g) for x, y, m, s, g in zip(xs, ys, maps, sig, gam)])
+        self._keep = (ks, list(xs), list(ys), list(maps), list(ests))
+        arr = (LpaImage * n)(*[image_struct(x, y, m, s, g, None, e)
+                               for x, y, m, s, g, e in zip(xs, ys, maps, sig, gam, ests)])
         h = ctypes.c_void_p()
         check(lib().lpa_plan_create(ks.handle, arr, n, int(flags), ctypes.byref(h)))
         self._h = h

@@ -43,6 +43,7 @@ class LpaImage(ctypes.Structure):
         ("sigma", ctypes.c_float),
         ("gamma", ctypes.c_float),
         ("variances", ctypes.c_void_p),
+        ("estimates", ctypes.c_void_p),
     ]
 
 

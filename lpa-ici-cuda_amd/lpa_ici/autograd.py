@@ -3,10 +3,10 @@
 LpaIciFunction.apply(x, taps, template, sigma, gamma) denoises one device
 image x (H x W float32) with the kernel set whose run weights are `taps` (a
 1-D float32 tensor in the run-weight order of template.runs(), e.g.
-template_taps(template)), and returns y.  The forward keeps y and the uint8
-scale map; the backward calls lpa_ici_backward, which recomputes the
-per-direction estimates from that map (the selection stays frozen, so y is
-linear in x and rational in the taps through sigma^2 * sum(w^2)).
+template_taps(template)), and returns y.  The forward keeps y, the uint8
+scale map and the per-direction estimates at the selected scales; the
+backward calls lpa_ici_backward with them (the selection stays frozen, so y
+is linear in x and rational in the taps through sigma^2 * sum(w^2)).
 """
 import numpy as np
 import torch
@@ -38,15 +38,15 @@ class LpaIciFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, taps, template, sigma, gamma):
         ks = _kernel_set_with_taps(template, taps)
-        y, maps = denoise(ks, x.contiguous(), sigma, gamma, scale_maps=True)
-        ctx.save_for_backward(x, y, maps)
+        y, maps, est = denoise(ks, x.contiguous(), sigma, gamma, scale_maps=True, keep_estimates=True)
+        ctx.save_for_backward(x, y, maps, est)
         ctx.ks, ctx.sigma, ctx.gamma = ks, sigma, gamma
         return y
 
     @staticmethod
     def backward(ctx, gy):
-        x, y, maps = ctx.saved_tensors
-        dxs, dk = backward(ctx.ks, [x], [y], [maps], [gy.contiguous()], ctx.sigma, ctx.gamma)
+        x, y, maps, est = ctx.saved_tensors
+        dxs, dk = backward(ctx.ks, [x], [y], [maps], [gy.contiguous()], ctx.sigma, ctx.gamma, estimates=[est])
         return dxs[0], dk, None, None, None
 
 

@@ -114,6 +114,24 @@ def test_adjoint_ragged_width(ks, w, h):
     assert e_dx < 1e-4 and e_dk < 1e-4
 
 
+@pytest.mark.parametrize("w,h", [(72, 56), (200, 120)])
+def test_adjoint_with_kept_estimates(ks, w, h):
+    """The forward's kept estimates in place of the recomputation: dL/dx
+    bit-identical, dL/dkernel equal up to float reassociation of the
+    per-block partial rows."""
+    x = noisy_synth(w, h, seed=w)
+    if w == 200:
+        x[4:9, 6:12] = 4.0  # windows outside the fixed-point range too
+    xd = torch.from_numpy(x).cuda()
+    y, m, e = lpa_ici.denoise(ks, xd, SIGMA, GAMMA, scale_maps=True, keep_estimates=True)
+    g = torch.randn_like(xd)
+    dx0, dk0 = lpa_ici.backward(ks, [xd], [y], [m], [g], SIGMA, GAMMA)
+    dx1, dk1 = lpa_ici.backward(ks, [xd], [y], [m], [g], SIGMA, GAMMA, estimates=[e])
+    torch.cuda.synchronize()
+    assert torch.equal(dx0[0], dx1[0])
+    assert torch.allclose(dk0, dk1, rtol=1e-5, atol=1e-6 * dk0.abs().max().item())
+
+
 def test_adjoint_fixed_point_fallback(ks):
     """Values outside the fixed-point range in one corner: the windows within
     reach of them recompute yhat with float ring sums, as the forward does,

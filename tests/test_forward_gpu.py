@@ -152,6 +152,21 @@ def test_generated_wedge_sets(D, h):
     assert np.array_equal(y.view(np.uint32), yq.view(np.uint32))
 
 
+@pytest.mark.parametrize("flags,label", [(0, "fixed point"), (lpa_ici.FWD_EXACT, "exact"),
+                                         (lpa_ici.FWD_RING_FLOAT, "float ring"), (lpa_ici.FWD_WEIGHTED, "weighted")])
+def test_forward_keeps_estimates(ks, flags, label):
+    """The estimates a forward keeps are the values it aggregated: equal, bit
+    for bit, to lpa_ici_estimates at its scale maps; y and the maps do not
+    change when they are kept."""
+    xd = dev(noisy_synth(150, 90, seed=12))
+    y, m, e = lpa_ici.denoise(ks, xd, SIGMA, GAMMA, scale_maps=True, flags=flags, keep_estimates=True)
+    y0, m0 = lpa_ici.denoise(ks, xd, SIGMA, GAMMA, scale_maps=True, flags=flags)
+    e_ref = lpa_ici.estimates(ks, xd, m, flags=flags)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y0) and torch.equal(m, m0), label
+    assert torch.equal(e, e_ref), label
+
+
 def test_fixed_point_fallback_windows(ks):
     """A block of values outside the fixed-point range: windows that reach it
     take float ring sums, the rest stay in fixed point.  Far from the block
