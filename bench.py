@@ -14,11 +14,13 @@ carries
                 plain (one launch per image) vs one persistent launch vs its
                 hipGraph replay
   fwd_bwd       64 x 512^2 forward + adjoint (dL/dx, dL/dkernel); the forward
-                keeps the uint8 scale maps and each direction's selected
-                estimate, which the adjoint reads
-  config5       32 images with sides U{1024..8192} (about 680 Mpx), fwd+bwd
-                as above, and the peak device memory in use against the
-                288 GB of HBM
+                saves the uint8 scale maps and the adjoint recomputes the
+                estimates from them; timed again with the forward keeping
+                each direction's selected estimate for the adjoint to read
+                ("kept_estimates")
+  config5       32 images with sides U{1024..8192} (about 630 Mpx), fwd+bwd
+                in the same two forms, and each form's peak device memory in
+                use against the 288 GB of HBM
 Flags --skip-* drop the side legs.
 """
 import argparse
@@ -58,6 +60,12 @@ def timed(fn, steps, warmup, stream):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def rounded(d):
+    """A leg's record with its floats to three decimals, nested records too."""
+    return {k: rounded(v) if isinstance(v, dict) else round(v, 3) if isinstance(v, float) else v
+            for k, v in d.items()}
 
 
 def make_image(w, h, seed, sigma, noise_seed):
@@ -142,36 +150,53 @@ def batched_leg(ks, steps, warmup, stream, n_images=256, seed=3):
     return out
 
 
-def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
-    xs = [torch.from_numpy(make_image(side, side, 2000 + i, SIGMA, SEED + i)).cuda() for i in range(n)]
-    ys = [torch.empty_like(x) for x in xs]
-    maps = [torch.empty((ks.D, side, side), dtype=torch.uint8, device="cuda") for _ in xs]
-    # the forward keeps each direction's selected estimate for the adjoint
-    ests = [torch.empty((ks.D, side, side), dtype=torch.float32, device="cuda") for _ in xs]
-    g = torch.Generator(device="cuda").manual_seed(5)
-    dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
-    dxs = [torch.empty_like(x) for x in xs]
-    nw = ks.info()["num_weights"]
-    dk = torch.zeros(nw, dtype=torch.float32, device="cuda")
-    fwd = lpa_ici.Plan(ks, xs, ys, maps, SIGMA, GAMMA, flags=0, estimates=ests)
+def adjoint_variant(ks, xs, ys, maps, sig, gam, dys, dxs, dk, steps, warmup, stream, variant, on_step=None):
+    """Times forward and forward + adjoint over the images, in one of two
+    forms: "recompute" as the workload states it (the forward saves only the
+    uint8 scale maps and the adjoint recomputes each direction's estimate from
+    them), or "kept_estimates" (the forward also keeps each direction's
+    selected estimate, 16 float maps per image, for the adjoint to read).
+    Returns (fwd_ms, fwd_bwd_ms)."""
     import ctypes
-    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, SIGMA, GAMMA, None, e)
-                                   for x, y, m, e in zip(xs, ys, maps, ests)])
+    n = len(xs)
+    sig = sig if isinstance(sig, list) else [sig] * n
+    gam = gam if isinstance(gam, list) else [gam] * n
     dy_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dys])
     dx_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dxs])
     sp = ctypes.c_void_p(stream.cuda_stream)
-
-    def bwd():
-        lpa_ici.check(lpa_ici.lib().lpa_ici_backward(ks.handle, arr, dy_p, dx_p, ctypes.c_void_p(dk.data_ptr()), n,
-                                                     sp))
+    ests = ([torch.empty((ks.D,) + tuple(x.shape), dtype=torch.float32, device="cuda") for x in xs]
+            if variant == "kept_estimates" else None)
+    fwd = lpa_ici.Plan(ks, xs, ys, maps, sig, gam, flags=0, estimates=ests)
+    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, s_, g_, None, ests[i] if ests else None)
+                                   for i, (x, y, m, s_, g_) in enumerate(zip(xs, ys, maps, sig, gam))])
 
     def step():
         fwd.run(stream)
-        bwd()
+        lpa_ici.check(lpa_ici.lib().lpa_ici_backward(ks.handle, arr, dy_p, dx_p, ctypes.c_void_p(dk.data_ptr()),
+                                                     n, sp))
+        if on_step:
+            on_step()
 
     with torch.cuda.stream(stream):
         ms_f = timed(lambda: fwd.run(stream), steps, warmup, stream)
         ms_fb = timed(step, steps, warmup, stream)
+    if on_step:
+        torch.cuda.synchronize()
+        on_step()
+    fwd.close()
+    return ms_f, ms_fb
+
+
+def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
+    xs = [torch.from_numpy(make_image(side, side, 2000 + i, SIGMA, SEED + i)).cuda() for i in range(n)]
+    ys = [torch.empty_like(x) for x in xs]
+    maps = [torch.empty((ks.D, side, side), dtype=torch.uint8, device="cuda") for _ in xs]
+    g = torch.Generator(device="cuda").manual_seed(5)
+    dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
+    dxs = [torch.empty_like(x) for x in xs]
+    dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
+    times = {v: adjoint_variant(ks, xs, ys, maps, SIGMA, GAMMA, dys, dxs, dk, steps, warmup, stream, v)
+             for v in ("recompute", "kept_estimates")}
     npx = n * side * side
     # Algorithmic adjoint work: per selected kernel tap of every pixel and
     # direction, 2 FLOP to recompute yhat, 2 for the dL/dx gather and 4 for
@@ -179,14 +204,20 @@ def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
     nnz = torch.from_numpy(ks.runs()["nnz"].astype(np.int64).reshape(ks.D, ks.S)).cuda()
     dsel = torch.arange(ks.D, device="cuda")[:, None, None]
     sel_taps = int(sum(nnz[dsel, m.long()].sum().item() for m in maps))
-    ms_b = ms_fb - ms_f
-    bwd_tflops = 8.0 * sel_taps / (ms_b * 1e-3) / 1e12
-    log(f"[fwd+bwd] {n}x{side}^2: fwd {ms_f:.2f} ms, fwd+bwd {ms_fb:.2f} ms, {npx / ms_fb / 1e3:.1f} Mpx/s; "
-        f"adjoint {ms_b:.2f} ms, {sel_taps / npx:.0f} selected taps/px, {bwd_tflops:.1f} TFLOP/s algorithmic")
-    fwd.close()
-    return {"images": n, "side": side, "fwd_ms": ms_f, "fwd_bwd_ms": ms_fb, "fwd_mpx_s": npx / ms_f / 1e3,
-            "fwd_bwd_mpx_s": npx / ms_fb / 1e3, "bwd_ms": ms_b, "selected_taps_per_px": sel_taps / npx,
-            "bwd_tflops_algorithmic": bwd_tflops}
+    rec = {"images": n, "side": side, "selected_taps_per_px": sel_taps / npx}
+    for variant, (ms_f, ms_fb) in times.items():
+        ms_b = ms_fb - ms_f
+        bwd_tflops = 8.0 * sel_taps / (ms_b * 1e-3) / 1e12
+        log(f"[fwd+bwd {variant}] {n}x{side}^2: fwd {ms_f:.2f} ms, fwd+bwd {ms_fb:.2f} ms, "
+            f"{npx / ms_fb / 1e3:.1f} Mpx/s; adjoint {ms_b:.2f} ms, {sel_taps / npx:.0f} selected taps/px, "
+            f"{bwd_tflops:.1f} TFLOP/s algorithmic")
+        v = {"fwd_ms": ms_f, "fwd_bwd_ms": ms_fb, "fwd_mpx_s": npx / ms_f / 1e3, "fwd_bwd_mpx_s": npx / ms_fb / 1e3,
+             "bwd_ms": ms_b, "bwd_tflops_algorithmic": bwd_tflops}
+        if variant == "recompute":
+            rec.update(v)
+        else:
+            rec[variant] = v
+    return rec
 
 
 def measured_traffic(walk):
@@ -204,9 +235,9 @@ def measured_traffic(walk):
 
 def config5_leg(ks, steps, warmup, stream, n=32, seed=5):
     """32 images with sides U{1024..8192}, per-image sigma/gamma as the batched
-    leg: forward (scale maps kept as uint8, per-direction estimates as
-    float32) + adjoint that reads them; peak HBM use reported."""
-    import ctypes
+    leg: forward (scale maps kept as uint8) + adjoint that recomputes the
+    estimates from them, then the same with the estimates kept; peak device
+    memory in use reported."""
     rng = np.random.default_rng(seed)
     sides = rng.integers(1024, 8193, size=(n, 2))
     sig = [float(rng.choice([10, 20, 30, 40])) / 255.0 for _ in range(n)]
@@ -221,40 +252,38 @@ def config5_leg(ks, steps, warmup, stream, n=32, seed=5):
     log(f"[config5] generated {n} images, {npx / 1e6:.1f} Mpx in {time.time() - t0:.1f} s")
     ys = [torch.empty_like(x) for x in xs]
     maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda") for x in xs]
-    ests = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.float32, device="cuda") for x in xs]
     g = torch.Generator(device="cuda").manual_seed(7)
     dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
     dxs = [torch.empty_like(x) for x in xs]
     dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
-    fwd = lpa_ici.Plan(ks, xs, ys, maps, sig, gam, flags=0, estimates=ests)
-    arr = (lpa_ici.LpaImage * n)(*[lpa_ici.image_struct(x, y, m, s_, g_, None, e) for x, y, m, s_, g_, e in
-                                   zip(xs, ys, maps, sig, gam, ests)])
-    dy_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dys])
-    dx_p = (ctypes.c_void_p * n)(*[d.data_ptr() for d in dxs])
-    sp = ctypes.c_void_p(stream.cuda_stream)
-    free_min = [free0]
+    free_min = {"recompute": free0, "kept_estimates": free0}
+    current = ["recompute"]
 
-    def step():
-        fwd.run(stream)
-        lpa_ici.check(lpa_ici.lib().lpa_ici_backward(ks.handle, arr, dy_p, dx_p, ctypes.c_void_p(dk.data_ptr()),
-                                                     n, sp))
-        free_min[0] = min(free_min[0], torch.cuda.mem_get_info()[0])
+    def sample():
+        free_min[current[0]] = min(free_min[current[0]], torch.cuda.mem_get_info()[0])
 
-    with torch.cuda.stream(stream):
-        ms_f = timed(lambda: fwd.run(stream), steps, warmup, stream)
-        ms_fb = timed(step, steps, warmup, stream)
-    torch.cuda.synchronize()
-    free_min[0] = min(free_min[0], torch.cuda.mem_get_info()[0])
-    fwd.close()
-    peak = free0 - free_min[0]
-    log(f"[config5] {n} images {npx / 1e6:.1f} Mpx: fwd {ms_f:.1f} ms ({npx / ms_f / 1e3:.0f} Mpx/s), "
-        f"fwd+bwd {ms_fb:.1f} ms ({npx / ms_fb / 1e3:.0f} Mpx/s); peak device memory in use "
-        f"{peak / 2**30:.1f} GiB of {total / 2**30:.0f} GiB")
-    return {"images": n, "mpx": round(npx / 1e6, 1), "fwd_ms": ms_f, "fwd_mpx_s": npx / ms_f / 1e3,
-            "fwd_bwd_ms": ms_fb, "fwd_bwd_mpx_s": npx / ms_fb / 1e3, "peak_hbm_gib": round(peak / 2**30, 2),
-            "hbm_total_gib": round(total / 2**30, 1),
-            "torch_peak_allocated_gib": round(torch.cuda.max_memory_allocated() / 2**30, 2),
-            "scale_maps_gib": round(npx * ks.D / 2**30, 2)}
+    # the recompute form runs first, before any estimates exist, so each
+    # variant's peak is its own
+    times = {}
+    for variant in ("recompute", "kept_estimates"):
+        current[0] = variant
+        times[variant] = adjoint_variant(ks, xs, ys, maps, sig, gam, dys, dxs, dk, steps, warmup, stream, variant,
+                                         sample)
+    rec = {"images": n, "mpx": round(npx / 1e6, 1), "hbm_total_gib": round(total / 2**30, 1),
+           "scale_maps_gib": round(npx * ks.D / 2**30, 2)}
+    for variant, (ms_f, ms_fb) in times.items():
+        peak = free0 - free_min[variant]
+        log(f"[config5 {variant}] {n} images {npx / 1e6:.1f} Mpx: fwd {ms_f:.1f} ms ({npx / ms_f / 1e3:.0f} Mpx/s), "
+            f"fwd+bwd {ms_fb:.1f} ms ({npx / ms_fb / 1e3:.0f} Mpx/s); peak device memory in use "
+            f"{peak / 2**30:.1f} GiB of {total / 2**30:.0f} GiB")
+        v = {"fwd_ms": ms_f, "fwd_mpx_s": npx / ms_f / 1e3, "fwd_bwd_ms": ms_fb, "fwd_bwd_mpx_s": npx / ms_fb / 1e3,
+             "peak_hbm_gib": round(peak / 2**30, 2)}
+        if variant == "recompute":
+            rec.update(v)
+        else:
+            rec[variant] = v
+    rec["torch_peak_allocated_gib"] = round(torch.cuda.max_memory_allocated() / 2**30, 2)
+    return rec
 
 
 def cpu_baseline(x_host, crop=640):
@@ -409,10 +438,10 @@ def main():
             rec["batched_uneven"][k] = {kk: round(vv, 3) for kk, vv in b[k].items()}
     if not args.skip_fwdbwd:
         fb = fwd_bwd_leg(ks, max(3, args.steps // 4), 1, stream)
-        rec["fwd_bwd"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in fb.items()}
+        rec["fwd_bwd"] = rounded(fb)
     if not args.skip_config5:
         c5 = config5_leg(ks, 2, 1, stream)
-        rec["config5"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in c5.items()}
+        rec["config5"] = rounded(c5)
     rec["host"] = platform.node()
     print(json.dumps(rec), flush=True)
 
