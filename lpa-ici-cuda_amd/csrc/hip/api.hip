@@ -668,9 +668,9 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
         }
     };
     add_tiles(tiles, lpa::kBwdTileW, lpa::kBwdTileH);
-    add_tiles(dx_tiles, lpa::kDxTile, lpa::kDxTile);
+    add_tiles(dx_tiles, lpa::kDxTileW, ks->has_rings ? lpa::kDxTileRowsRing : lpa::kDxTileRowsWeighted);
     int bp = 1, bd = 1;
-    lpa::backward_occupancy(ks->radius, ks->S, K, &bp, &bd);
+    lpa::backward_occupancy(ks->radius, ks->has_rings, K, &bp, &bd);
     const int cus = num_cus();
     const int grid_prep = std::max(1, std::min((int)tiles.size(), cus * bp));
     const int grid_dx = std::max(1, std::min((int)dx_tiles.size(), cus * bd));

@@ -524,6 +524,11 @@ __device__ __forceinline__ void run_sums_sliding(const float *__restrict__ src, 
     }
 }
 
+// Ring sums of the masked map for the P pixels of a lane.  The switch has a
+// case per run length up to 16; a longer run (up to 32) takes its first 16
+// taps as a run of its own, which keeps the register window at 16 + P - 1.
+constexpr int kDxMaxCase = 16;
+static_assert(kMaxRun <= 2 * kDxMaxCase, "one split covers every run");
 template <int P>
 __device__ __forceinline__ void ring_sums_sliding(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
                                                   int2 kr, float (&acc)[P])
@@ -531,30 +536,46 @@ __device__ __forceinline__ void ring_sums_sliding(const float *__restrict__ lane
     for (int r = 0; r < kr.y; ++r) {
         const int4 run = load_const(runs + kr.x + r);
         const float *__restrict__ src = lane + __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
-        switch (run.z) {
+        int L = run.z;
+        if (L > kDxMaxCase) {
+            run_sums_sliding<kDxMaxCase, P>(src, acc);
+            src += kDxMaxCase;
+            L -= kDxMaxCase;
+        }
+        switch (L) {
 #define LPA_SLIDE_CASE(L)                                                                                          \
     case L: run_sums_sliding<L, P>(src, acc); break;
             LPA_SLIDE_CASE(1) LPA_SLIDE_CASE(2) LPA_SLIDE_CASE(3) LPA_SLIDE_CASE(4) LPA_SLIDE_CASE(5)
             LPA_SLIDE_CASE(6) LPA_SLIDE_CASE(7) LPA_SLIDE_CASE(8) LPA_SLIDE_CASE(9) LPA_SLIDE_CASE(10)
             LPA_SLIDE_CASE(11) LPA_SLIDE_CASE(12) LPA_SLIDE_CASE(13) LPA_SLIDE_CASE(14) LPA_SLIDE_CASE(15)
-            LPA_SLIDE_CASE(16) LPA_SLIDE_CASE(17) LPA_SLIDE_CASE(18) LPA_SLIDE_CASE(19) LPA_SLIDE_CASE(20)
-            LPA_SLIDE_CASE(21) LPA_SLIDE_CASE(22) LPA_SLIDE_CASE(23) LPA_SLIDE_CASE(24) LPA_SLIDE_CASE(25)
-            LPA_SLIDE_CASE(26) LPA_SLIDE_CASE(27) LPA_SLIDE_CASE(28) LPA_SLIDE_CASE(29) LPA_SLIDE_CASE(30)
-            LPA_SLIDE_CASE(31) LPA_SLIDE_CASE(32)
+            LPA_SLIDE_CASE(16)
 #undef LPA_SLIDE_CASE
         default: break;
         }
     }
 }
 
-// Pass 2 (dx), on 64 x 64 tiles with 1024 lanes (4 px per lane).  Each lane
-// keeps q and the scale index of 16 tile+halo elements in registers (rows
-// tid/64 + 16i, columns tid%64 + 64j); the LDS holds only the masked map and
-// the table.  The wide tile cuts the masked-map building per output pixel
-// (halo overhead (64+2R)^2/64^2 instead of (32+2R)^2/32^2).
-constexpr int DXW = 64, DXH = 64, DXP = 4, DXT = 1024, DXLX = DXW / DXP;
-static_assert(DXLX == 16, "the lane layout assumes 16 lanes per tile row");
-constexpr int kSlotRows = 8, kSlotCols = 2;  // 16 + 16i rows, 64 + 64j columns cover 126 x 126
+// Pass 2 (dx), on tiles 64 wide and 16 P tall, 1024 lanes of P pixels along
+// a row: P = 8 (64 x 128 tiles) for kernel sets in ring form, P = 4 (64 x 64)
+// for weighted sets, whose FMA window would not fit P = 8 in registers.  The
+// scalar work of a ring run (its record, the switch) is paid once per lane
+// whatever P, so the wider lane halves it per pixel.  Each lane keeps q and
+// the scale index of the tile+halo elements (rows tid/64 + 16i, columns
+// tid%64 + 64j) in registers; the LDS holds only the masked map.  A wave
+// covers P rows: each half-wave a 32-pixel-wide half of them (32 / P lanes
+// per row, P px apart, rows one bank apart: the 32 lanes of a read fall in
+// 32 banks).
+constexpr int DXW = 64, DXT = 1024;
+constexpr int DXLP = lds_pitch(DXW, kMaxRadius);  // the map's row pitch, 1 mod 32
+static_assert(DXLP % 32 == 1, "rows one bank apart");
+template <bool RING> struct DxShape {
+    static constexpr int P = RING ? 8 : 4;                   // pixels per lane, along a row
+    static constexpr int H = 16 * P;                         // tile rows: 16 waves of P rows
+    static constexpr int LH = 32 / P;                        // lanes per row in a half-wave
+    static constexpr int SLOT_ROWS = (H + 2 * 31 + 15) / 16;  // 16 + 16i rows cover H + 2R
+    static constexpr int SLOT_COLS = 2;                      // 64 + 64j columns cover 64 + 2R
+};
+static_assert(DxShape<true>::H == kDxTileRowsRing && DxShape<false>::H == kDxTileRowsWeighted, "tile lists");
 
 //
 // RING (kernel sets in ring form, k_{d,s} = w_{d,s} on support(d,s), supports
@@ -569,13 +590,21 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings,
                                                      const BwdDesc *__restrict__ imgs, const uint2 *__restrict__ tiles,
                                                      int num_tiles, int *__restrict__ counter, int R)
 {
+    using SH = DxShape<RING>;
+    constexpr int P = SH::P, DXH = SH::H, LH = SH::LH;
+    constexpr int SR = SH::SLOT_ROWS, SC = SH::SLOT_COLS;
+    static_assert(SR * SC <= 32, "slot validity bits fit one word");
+    // The map's row pitch is that of the largest radius, whatever R: a
+    // compile-time pitch puts every slot's LDS offset in the instruction.
+    constexpr int lp = DXLP;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ int next;
     const int S = ks.S, D = ks.D, K = D * S;
-    const int lp = lds_pitch(DXW, R);
     float *mt = (float *)lds;
+    float4 *mt4 = (float4 *)lds;
     const int tid = threadIdx.x;
     const int fw = DXW + 2 * R, fh = DXH + 2 * R;
+    const int map4 = (fh * lp + 3) / 4;  // float4s covering the map (the LDS size is rounded up to match)
     const int r0 = tid >> 6, c0 = tid & 63;
 
     for (;;) {
@@ -590,74 +619,88 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings,
         const float *__restrict__ inv_t = img.tab + 2 * K;  // wave-uniform scalar loads
         const size_t plane = (size_t)img.W * img.H;
 
-        float qv[kSlotRows][kSlotCols];
+        // Slot (i, j) of this lane is the window element at row r0 + 16 i,
+        // column c0 + 64 j, i.e. image offset lane_off + 16 i W + 64 j; bit
+        // i SC + j of `valid` says it lies in the window and in the image.
+        const int gx0 = x0 - R + c0, gy0 = y0 - R + r0;
+        const long long lane_off = (long long)gy0 * img.W + gx0;
+        unsigned valid = 0;
 #pragma unroll
-        for (int i = 0; i < kSlotRows; ++i)
+        for (int i = 0; i < SR; ++i)
 #pragma unroll
-            for (int j = 0; j < kSlotCols; ++j) {
-                const int r = r0 + 16 * i, c = c0 + 64 * j;
-                const int gy = y0 - R + r, gx = x0 - R + c;
+            for (int j = 0; j < SC; ++j) {
+                const int r = r0 + 16 * i, c = c0 + 64 * j, gy = gy0 + 16 * i, gx = gx0 + 64 * j;
                 const bool in = r < fh && c < fw && gy >= 0 && gy < img.H && gx >= 0 && gx < img.W;
-                qv[i][j] = in ? as_global(bd.q)[(size_t)gy * img.W + gx] : 0.0f;
+                valid |= (unsigned)in << (i * SC + j);
             }
-
-        // A wave covers 4 rows; each half-wave a 32-pixel-wide half of them
-        // (8 lanes per row, 4 px apart, rows one bank apart: 32 banks).
-        const int dl = tid & 63;
-        const int tx = (dl >> 5) * 8 + (dl & 7), ty = (tid >> 6) * 4 + ((dl & 31) >> 3);
-        const int px0 = x0 + tx * DXP, py = y0 + ty;
-        const float *lane_m = mt + (ty + R) * lp + tx * DXP + R;
-        float dx[DXP];
+        float qv[SR][SC];
 #pragma unroll
-        for (int p = 0; p < DXP; ++p) dx[p] = 0.0f;
+        for (int i = 0; i < SR; ++i)
+#pragma unroll
+            for (int j = 0; j < SC; ++j)
+                qv[i][j] = (valid >> (i * SC + j)) & 1u
+                               ? as_global(bd.q)[lane_off + (long long)16 * i * img.W + 64 * j]
+                               : 0.0f;
+
+        const int dl = tid & 63;
+        const int tx = (dl >> 5) * LH + (dl & (LH - 1)), ty = (tid >> 6) * P + ((dl & 31) / LH);
+        const int px0 = x0 + tx * P, py = y0 + ty;
+        const float *lane_m = mt + (ty + R) * lp + tx * P + R;
+        float dx[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) dx[p] = 0.0f;
 
         for (int d = 0; d < D; ++d) {
-            int sv[kSlotRows][kSlotCols];
+            // Recomputed per direction rather than held: the slot masks and
+            // addresses derived from these would otherwise stay live across
+            // the walk.
+            unsigned vb = valid;
+            long long off = lane_off;
+            asm volatile("" : "+v"(vb), "+v"(off));
+            const auto mp = as_global(img.map) + (size_t)d * plane;
+            int sv[SR][SC];
 #pragma unroll
-            for (int i = 0; i < kSlotRows; ++i)
+            for (int i = 0; i < SR; ++i)
 #pragma unroll
-                for (int j = 0; j < kSlotCols; ++j) {
-                    const int r = r0 + 16 * i, c = c0 + 64 * j;
-                    const int gy = y0 - R + r, gx = x0 - R + c;
-                    const bool in = r < fh && c < fw && gy >= 0 && gy < img.H && gx >= 0 && gx < img.W;
-                    sv[i][j] = in ? as_global(img.map)[(size_t)d * plane + (size_t)gy * img.W + gx] : 255;
-                }
+                for (int j = 0; j < SC; ++j)
+                    sv[i][j] = (vb >> (i * SC + j)) & 1u ? (int)mp[off + (long long)16 * i * img.W + 64 * j] : 255;
             if constexpr (RING) {
                 // The deepest scale selected in the window: the walk starts
                 // there, and every map from it down is non-empty.
                 int deepest = -1;
 #pragma unroll
-                for (int i = 0; i < kSlotRows; ++i)
+                for (int i = 0; i < SR; ++i)
 #pragma unroll
-                    for (int j = 0; j < kSlotCols; ++j)
+                    for (int j = 0; j < SC; ++j)
                         if (sv[i][j] < S && qv[i][j] != 0.0f) deepest = max(deepest, sv[i][j]);
-                const int top = block_max<DXT>(deepest);
+                const int top = block_max<DXT>(deepest);  // its first barrier: the last ring sums are done
+                if (top < 0) continue;
+                // Clear the map, then each step writes only the elements that
+                // join it; elements outside the window hold scale 255 and
+                // never join.
+                for (int e = tid; e < map4; e += DXT) mt4[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 for (int s = top; s >= 0; --s) {
-                    __syncthreads();  // previous ring sums done with the map
+                    __syncthreads();  // the clear, or the previous ring sums, done with the map
                     const int k = d * S + s;
                     const float cw = as_const(inv_t)[k] * as_const(rings.scale)[k];
 #pragma unroll
-                    for (int i = 0; i < kSlotRows; ++i)
+                    for (int i = 0; i < SR; ++i)
 #pragma unroll
-                        for (int j = 0; j < kSlotCols; ++j) {
-                            const int r = r0 + 16 * i, c = c0 + 64 * j;
-                            // the first step writes every element, later ones
-                            // only those that join the map
-                            if (r < fh && c < fw && (s == top || sv[i][j] == s))
-                                mt[r * lp + c] = sv[i][j] == s ? qv[i][j] * cw : 0.0f;
-                        }
+                        for (int j = 0; j < SC; ++j)
+                            if (sv[i][j] == s) mt[(r0 + 16 * i) * lp + c0 + 64 * j] = qv[i][j] * cw;
                     __syncthreads();
-                    ring_sums_sliding<DXP>(lane_m, lp, rings.flip, load_const(rings.kern + k), dx);
+                    ring_sums_sliding<P>(lane_m, lp, rings.flip, load_const(rings.kern + k), dx);
                 }
+                // the next direction's block_max is the barrier before its clear
             } else {
                 for (int s = 0; s < S; ++s) {
                     bool any = false;
                     __syncthreads();  // previous correlation done with the map
                     const float inv = as_const(inv_t)[d * S + s];
 #pragma unroll
-                    for (int i = 0; i < kSlotRows; ++i)
+                    for (int i = 0; i < SR; ++i)
 #pragma unroll
-                        for (int j = 0; j < kSlotCols; ++j) {
+                        for (int j = 0; j < SC; ++j) {
                             const int r = r0 + 16 * i, c = c0 + 64 * j;
                             if (r < fh && c < fw) {
                                 const bool m = sv[i][j] == s && qv[i][j] != 0.0f;
@@ -666,13 +709,13 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings,
                             }
                         }
                     if (!block_vote<DXT, false>(any)) continue;
-                    correlate<DXP, false, 0>(lane_m, lp, adj.flip, d * S + s, dx);
+                    correlate<P, false, 0>(lane_m, lp, adj.flip, d * S + s, dx);
                 }
             }
         }
         const auto out = as_global(bd.dx) + (size_t)py * img.pitch + px0;
 #pragma unroll
-        for (int p = 0; p < DXP; ++p)
+        for (int p = 0; p < P; ++p)
             if (py < img.H && px0 + p < img.W) out[p] = dx[p];
         __syncthreads();  // LDS and `next` are reused by the next tile
     }
@@ -721,10 +764,27 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(const double *__restr
 }
 
 size_t grad_lds_bytes(int R, int K) { return sizeof(float) * ((size_t)grad_tab_offset(R) + 3 * K); }
-size_t dx_lds_bytes(int R, int K)
+size_t dx_lds_bytes(int R, bool ring)
 {
-    (void)K;
-    return sizeof(float) * (size_t)lds_floats(DXW, DXH, R);
+    // (H + 2R) rows of pitch DXLP, in whole float4s: the ring form clears the
+    // map 16 bytes at a time
+    const size_t floats = (size_t)((ring ? DxShape<true>::H : DxShape<false>::H) + 2 * R) * DXLP;
+    return sizeof(float4) * ((floats + 3) / 4);
+}
+
+// The dx pass's LDS (the 64 x 128 map of the ring form) can pass the default
+// limit of a launch: raise it.
+hipError_t allow_dx_lds(int R)
+{
+    for (bool ring : {true, false}) {
+        const size_t bytes = dx_lds_bytes(R, ring);
+        if (bytes <= 65536) continue;
+        const void *f = ring ? reinterpret_cast<const void *>(&bwd_dx_kernel<true>)
+                             : reinterpret_cast<const void *>(&bwd_dx_kernel<false>);
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // The first pass's LDS can pass the default limit of a launch: raise it.
@@ -742,18 +802,19 @@ hipError_t allow_grad_lds(int R, int K)
 
 }  // namespace
 
-void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, int *dx)
+void backward_occupancy(int radius, bool ring, int num_kernels, int *prep, int *dx)
 {
     int b = 0;
-    (void)num_scales;
     if (allow_grad_lds(radius, num_kernels) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_grad_kernel<true>, GNT,
                                                      grad_lds_bytes(radius, num_kernels)) != hipSuccess)
         b = 1;
     *prep = b > 0 ? b : 1;
     b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bwd_dx_kernel<true>, DXT, dx_lds_bytes(radius, num_kernels)) !=
-        hipSuccess)
+    const void *f = ring ? reinterpret_cast<const void *>(&bwd_dx_kernel<true>)
+                         : reinterpret_cast<const void *>(&bwd_dx_kernel<false>);
+    if (allow_dx_lds(radius) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, f, DXT, dx_lds_bytes(radius, ring)) != hipSuccess)
         b = 1;
     *dx = b > 0 ? b : 1;
 }
@@ -765,9 +826,11 @@ hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAd
 {
     const int K = ks.D * ks.S, NW = adj.num_weights;
     if (ks.S > kMaxScales) return hipErrorInvalidValue;
-    if (grad_lds_bytes(radius, K) > (size_t)kMaxLdsBytes || dx_lds_bytes(radius, K) > 65536)
+    const bool ring = rings.runs != nullptr;
+    if (grad_lds_bytes(radius, K) > (size_t)kMaxLdsBytes || dx_lds_bytes(radius, ring) > (size_t)kMaxLdsBytes)
         return hipErrorInvalidValue;
-    const hipError_t ea = allow_grad_lds(radius, K);
+    hipError_t ea = allow_grad_lds(radius, K);
+    if (ea == hipSuccess) ea = allow_dx_lds(radius);
     if (ea != hipSuccess) return ea;
     const int rows = grid_prep;
     const size_t row_floats = (size_t)rows * (NW + K);
@@ -779,12 +842,12 @@ hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAd
     else
         hipLaunchKernelGGL(bwd_grad_kernel<false>, dim3(grid_prep), dim3(GNT), grad_lds_bytes(radius, K), stream,
                            ks, rings, adj, d_imgs, d_tiles, num_tiles, d_counter, radius, partial);
-    if (rings.runs)
-        hipLaunchKernelGGL(bwd_dx_kernel<true>, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, rings,
-                           adj, d_imgs, dx_tiles, num_dx_tiles, d_counter, radius);
+    if (ring)
+        hipLaunchKernelGGL(bwd_dx_kernel<true>, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, true), stream, ks,
+                           rings, adj, d_imgs, dx_tiles, num_dx_tiles, d_counter, radius);
     else
-        hipLaunchKernelGGL(bwd_dx_kernel<false>, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, K), stream, ks, rings,
-                           adj, d_imgs, dx_tiles, num_dx_tiles, d_counter, radius);
+        hipLaunchKernelGGL(bwd_dx_kernel<false>, dim3(grid_dx), dim3(DXT), dx_lds_bytes(radius, false), stream, ks,
+                           rings, adj, d_imgs, dx_tiles, num_dx_tiles, d_counter, radius);
     if (dkernel) {
         // NW + K doubles after the rows, at the next 8-byte boundary
         double *sums = (double *)(partial + ((row_floats + 1) & ~(size_t)1));

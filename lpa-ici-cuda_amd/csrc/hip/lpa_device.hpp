@@ -86,12 +86,12 @@ constexpr int kMaxLdsBytes = 160 * 1024;
 // LDS row pitch for a tile of width tw and halo radius r: at least tw + 2r and
 // congruent to pm mod 32 (default 1: lanes four pixels apart on consecutive
 // rows fall in different banks).
-__host__ __device__ inline int lds_pitch(int tw, int r, int pm = 1)
+__host__ __device__ constexpr int lds_pitch(int tw, int r, int pm = 1)
 {
-    int p = tw + 2 * r;
+    const int p = tw + 2 * r;
     return p + ((pm - p) % 32 + 32) % 32;
 }
-__host__ __device__ inline int lds_floats(int tw, int th, int r, int pm = 1)
+__host__ __device__ constexpr int lds_floats(int tw, int th, int r, int pm = 1)
 {
     return (th + 2 * r) * lds_pitch(tw, r, pm);
 }
@@ -150,12 +150,13 @@ hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &ri
                             hipStream_t stream);
 
 // Adjoint (lpa_bwd.hip).  The first pass walks 64 x 64 tiles (d_tiles), the
-// dx pass 64 x 64 tiles (dx_tiles); both queues are laid out as for the
+// dx pass tiles 64 wide and kDxTileRowsRing (ring form) or
+// kDxTileRowsWeighted tall (dx_tiles); both queues are laid out as for the
 // batched forward ({image, (tile_y << 16) | tile_x}).
 //   partial: [grid_prep][num_weights + K] floats (+1+2(num_weights+K) for the column sums in double), one row
 //            per work-group of the first pass, accumulated by its waves
 //   dkernel: += reduction of the rows plus the variance term (may be null)
-constexpr int kBwdTileW = 64, kBwdTileH = 64, kDxTile = 64;
+constexpr int kBwdTileW = 64, kBwdTileH = 64, kDxTileW = 64, kDxTileRowsRing = 128, kDxTileRowsWeighted = 64;
 // Row pitch residue (mod 32 banks) of the first pass's float tile: with rows
 // 9 banks apart, the 64 taps a wave reads together when the taps of a kernel
 // are on the lanes spread over the banks for every direction of the shipped
@@ -168,7 +169,7 @@ hipError_t launch_backward(const DevKset &ks, const DevRings &rings, const DevAd
                            int num_tiles, const uint2 *dx_tiles, int num_dx_tiles, int *d_counter, int radius,
                            int grid_prep, int grid_dx, float *partial, float *dkernel, hipStream_t stream);
 // Work-groups per CU of the two tile passes.
-void backward_occupancy(int radius, int num_scales, int num_kernels, int *prep, int *dx);
+void backward_occupancy(int radius, bool ring, int num_kernels, int *prep, int *dx);
 
 }  // namespace lpa
 
