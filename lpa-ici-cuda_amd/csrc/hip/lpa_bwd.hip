@@ -202,8 +202,10 @@ __device__ __forceinline__ void load_scales(const uint8_t *__restrict__ m, bool 
 // with the pixels, not with the wave: one LDS read and one FMA per pixel and
 // tap, against the tile walk's sliding window over all 512 pixels of the wave.
 // The tap offsets in the tile are read from tap_lds, one load per slice.
+// `pix_off` is the tile offset of the lane's first pixel from base0
+// (grad_lane_row * lp + grad_lane_col * GP): a pixel's offset is one readlane.
 template <int IT>
-__device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int base0, int lp,
+__device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int base0, int pix_off,
                                               const int *__restrict__ tap_lds, int w0, int taps, const float (&m)[GP],
                                               const int (&sd)[GP], int s, float *__restrict__ row, int lane)
 {
@@ -219,7 +221,7 @@ __device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int 
     // lane L holds pixels grad_lane_col(L) * GP + p of row grad_lane_row(L) of the wave
     auto pixel = [&](int L, int p, float &c) {
         c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m[p]), L));
-        return base0 + grad_lane_row(L) * lp + grad_lane_col(L) * GP + p;
+        return base0 + __builtin_amdgcn_readlane(pix_off, L) + p;
     };
 #pragma unroll
     for (int p = 0; p < GP; ++p) {
@@ -454,11 +456,12 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                     const int slices = (kw.y + 63) >> 6;
                     if (slices <= kMaxLaneSlices && n * slices * kLanePixelCost < kw.z * kPieceCost) {
                         const int base0 = (wave * (64 / GLX) + R) * lp + R;
+                        const int po = grad_lane_row(lane) * lp + grad_lane_col(lane) * GP;
                         switch (slices) {
-                        case 1: taps_on_lanes<1>(xt, base0, lp, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
-                        case 2: taps_on_lanes<2>(xt, base0, lp, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
-                        case 3: taps_on_lanes<3>(xt, base0, lp, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
-                        default: taps_on_lanes<4>(xt, base0, lp, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
+                        case 1: taps_on_lanes<1>(xt, base0, po, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
+                        case 2: taps_on_lanes<2>(xt, base0, po, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
+                        case 3: taps_on_lanes<3>(xt, base0, po, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
+                        default: taps_on_lanes<4>(xt, base0, po, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
                         }
                         continue;
                     }
