@@ -34,13 +34,17 @@ constexpr int kMaxScales = 16;
 // held to 128 registers (two work-groups per CU).
 constexpr int GW = kBwdTileW, GH = kBwdTileH, GP = 8, GLX = GW / GP, GNT = GLX * GH;
 static_assert(GNT % 64 == 0 && GNT <= 1024, "pass-1 lanes: whole waves, at most 1024");
-// A wave covers 8 rows of the tile.  Each half-wave (whose 32 lanes the LDS
-// serves together) takes a 32-pixel-wide half of them: 4 lanes per row, 8 px
-// apart, on rows 9 (float tile) or 1 (prefix table) banks apart, so the 32
-// lanes of a read hit 32 distinct banks.
-static_assert(GLX == 8, "the lane layout assumes 8 lanes per tile row");
-__device__ __forceinline__ int grad_lane_col(int lane) { return (lane >> 5) * 4 + (lane & 3); }
-__device__ __forceinline__ int grad_lane_row(int lane) { return (lane & 31) >> 2; }
+// A wave covers a 16 x 32 block of the tile (the 8 waves: 2 across, 4 down),
+// which holds fewer distinct selected scales than a 8 x 64 strip, so fewer
+// kernels are walked per wave.  Each half-wave (whose 32 lanes the LDS serves
+// together) takes 8 rows of it: 4 lanes per row, 8 px apart, on rows 9 (float
+// tile) or 1 (prefix table) banks apart, so the 32 lanes of a read hit 32
+// distinct banks.  Columns are in units of GP pixels.
+static_assert(GLX == 8 && GNT == 512, "the lane layout assumes 8 waves on a 64 x 64 tile");
+__device__ __forceinline__ int grad_lane_col(int lane) { return lane & 3; }
+__device__ __forceinline__ int grad_lane_row(int lane) { return (lane >> 5) * 8 + ((lane & 31) >> 2); }
+__device__ __forceinline__ int grad_wave_col(int wave) { return (wave & 1) * 4; }
+__device__ __forceinline__ int grad_wave_row(int wave) { return (wave >> 1) * 16; }
 
 // The float tile (pitch lds_pitch(GW, R, kBwdPitchResidue)) and the prefix
 // table (one more column, pitch lds_pitch(GW + 1, R)) share the start of the
@@ -218,7 +222,7 @@ __device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int 
     float acc[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) acc[i] = 0.0f;
-    // lane L holds pixels grad_lane_col(L) * GP + p of row grad_lane_row(L) of the wave
+    // lane L holds pixels grad_lane_col(L) * GP + p of row grad_lane_row(L) of the wave's block
     auto pixel = [&](int L, int p, float &c) {
         c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m[p]), L));
         return base0 + __builtin_amdgcn_readlane(pix_off, L) + p;
@@ -329,7 +333,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
         }
 
         const float *inv_t = tab + 2 * K;
-        const int tx = grad_lane_col(lane), ty = wave * (64 / GLX) + grad_lane_row(lane);
+        const int tx = grad_wave_col(wave) + grad_lane_col(lane), ty = grad_wave_row(wave) + grad_lane_row(lane);
         const int px0 = x0 + tx * GP, py = y0 + ty;
         const float *lane_x = xt + (ty + R) * lp + tx * GP + R;
         const size_t plane = (size_t)img.W * img.H;
@@ -455,7 +459,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                     const int4 kw = load_const(adj.kern_w + k);
                     const int slices = (kw.y + 63) >> 6;
                     if (slices <= kMaxLaneSlices && n * slices * kLanePixelCost < kw.z * kPieceCost) {
-                        const int base0 = (wave * (64 / GLX) + R) * lp + R;
+                        const int base0 = (grad_wave_row(wave) + R) * lp + grad_wave_col(wave) * GP + R;
                         const int po = grad_lane_row(lane) * lp + grad_lane_col(lane) * GP;
                         switch (slices) {
                         case 1: taps_on_lanes<1>(xt, base0, po, adj.tap_lds, kw.x, kw.y, m, sd, s, row, lane); break;
