@@ -110,20 +110,20 @@ __device__ __forceinline__ float wave_total(float v)
     return row_total(c + d);
 }
 
-// Sum L (<= 8) values over the 64 lanes of a wave and add the L totals into
-// row[0..L) with fire-and-forget float atomics.  Reduce-scatter: the swaps
+// Sum L (<= 8) values over the 64 lanes of a wave and add the first n (<= L)
+// totals into row[0..n) with fire-and-forget float atomics.  Reduce-scatter: the swaps
 // halve the values per lane across lane bits 5 and 4 without any select, so
 // each lane keeps Lp/4 partial sums (Lp = L rounded up to a power of two),
 // which the row reductions finish; lanes 0, 16, 32 and 48 own
 // j = b5 * Lp/2 + b4 * Lp/4 + i.
 template <int L>
-__device__ __forceinline__ void reduce_scatter_add(const float (&v)[L], float *__restrict__ row, int lane)
+__device__ __forceinline__ void reduce_scatter_add(const float (&v)[L], float *__restrict__ row, int lane, int n = L)
 {
     static_assert(L >= 1 && L <= 8, "pieces of at most 8 taps");
     constexpr int Lp = L <= 1 ? 1 : L <= 2 ? 2 : L <= 4 ? 4 : 8;
     if constexpr (Lp == 1) {
         const float t = wave_total(v[0]);
-        if (lane == 0) unsafeAtomicAdd(row, t);
+        if (lane == 0 && n > 0) unsafeAtomicAdd(row, t);
     } else if constexpr (Lp == 2) {
         float a = v[0], b = L > 1 ? v[1] : 0.0f;
         swap32(a, b);
@@ -131,7 +131,7 @@ __device__ __forceinline__ void reduce_scatter_add(const float (&v)[L], float *_
         swap16(c, d);
         const float t = row_total(c + d);
         const int j = lane >> 5;
-        if ((lane & 31) == 0 && j < L) unsafeAtomicAdd(row + j, t);
+        if ((lane & 31) == 0 && j < n) unsafeAtomicAdd(row + j, t);
     } else {
         float h[Lp / 2];
 #pragma unroll
@@ -146,7 +146,7 @@ __device__ __forceinline__ void reduce_scatter_add(const float (&v)[L], float *_
             swap16(a, b);
             const float t = row_total(a + b);
             const int j = (lane >> 5) * (Lp / 2) + ((lane >> 4) & 1) * (Lp / 4) + i;
-            if ((lane & 15) == 0 && j < L) unsafeAtomicAdd(row + j, t);
+            if ((lane & 15) == 0 && j < n) unsafeAtomicAdd(row + j, t);
         }
     }
 }
@@ -411,20 +411,30 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                         }
                     }
                 }
-                // E_k: sum over the pixels that selected s of c (yhat - y).
-                for (int s = 0; s < S; ++s) {
-                    bool want = false;
+                // E_k: sum over the pixels that selected s of c (yhat - y),
+                // c = q inv_k.  Each lane sums its pixels per scale; the wave
+                // then sums 8 scales at a time with one reduce-scatter.
+                float dv[GP];
 #pragma unroll
-                    for (int p = 0; p < GP; ++p) want |= sd[p] == s;
-                    if (!__any(want)) continue;
-                    const int k = d * S + s;
-                    const float inv = inv_t[k];
-                    float e = 0.0f;
+                for (int p = 0; p < GP; ++p) dv[p] = yh[p] - yv[p];
+                for (int s0 = 0; s0 < S; s0 += 8) {
+                    float e[8];
 #pragma unroll
-                    for (int p = 0; p < GP; ++p)
-                        if (sd[p] == s) e = __builtin_fmaf(q[p] * inv, yh[p] - yv[p], e);
-                    e = wave_total(e);
-                    if (lane == 0) unsafeAtomicAdd(row + NW + k, e);
+                    for (int j = 0; j < 8; ++j) {
+                        const int s = s0 + j;
+                        bool want = false;
+#pragma unroll
+                        for (int p = 0; p < GP; ++p) want |= sd[p] == s;
+                        float a = 0.0f;
+                        if (__any(want)) {
+#pragma unroll
+                            for (int p = 0; p < GP; ++p)
+                                if (sd[p] == s) a = __builtin_fmaf(q[p], dv[p], a);
+                            a *= inv_t[d * S + s];
+                        }
+                        e[j] = a;
+                    }
+                    reduce_scatter_add<8>(e, row + NW + d * S + s0, lane, min(8, S - s0));
                 }
             }
             if (fixed) {
