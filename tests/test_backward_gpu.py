@@ -105,6 +105,32 @@ def test_adjoint_weighted_kernel_set():
     assert e_dx < 1e-4 and e_dk < 1e-4
 
 
+@pytest.mark.parametrize("D,h", [(8, [1, 2, 4, 8]), (32, [1, 3, 6, 12, 24]), (4, [1, 2, 3, 5, 8, 13, 21, 32]),
+                                 (8, [1, 2, 3, 4, 5, 6, 8, 10, 12, 16])])
+def test_adjoint_generated_wedge_sets(D, h):
+    """Generated ring-form sets with other direction counts, radii (7 to 31)
+    and scale counts (up to 10, so E takes two groups of 8 scales): the dx
+    pass's window and both passes' walks at these shapes, on an image whose
+    sides are not multiples of the tiles."""
+    g = lpa_ici.wedge_kernels(D, h)
+    assert g.info()["ring_form"] == 1
+    e_dx, e_dk = adjoint_error(g, noisy_synth(150, 140, seed=D + len(h)))
+    print(f"D={D} h={h} vs FP64: dx rel {e_dx:.2e}, dk rel {e_dk:.2e}")
+    assert e_dx < 1e-4 and e_dk < 1e-4
+
+
+@pytest.mark.parametrize("name", ["lpa_kernels_m_1_0_d_8_h12_1_2_4_8.txt", "lpa_kernels_m_1_0_d_16_h12_1_2_4_8_16.txt",
+                                  "lpa_kernels_m_1_0_d_8_h12_1_15_15_30.txt", "lpa_kernels_m_0_0_d_4_square.txt"])
+def test_adjoint_shipped_sets(name):
+    """Shipped sets with radii 7, 15 and 29, weighted (the dx pass's 4-pixel
+    lanes on 64 x 64 tiles) and in ring form (a square set), against the FP64
+    adjoint."""
+    k = kernels(name)
+    e_dx, e_dk = adjoint_error(k, noisy_synth(150, 140, seed=7))
+    print(f"{name} vs FP64: dx rel {e_dx:.2e}, dk rel {e_dk:.2e}")
+    assert e_dx < 1e-4 and e_dk < 1e-4
+
+
 @pytest.mark.parametrize("w,h", [(70, 45), (131, 9)])
 def test_adjoint_ragged_width(ks, w, h):
     """Widths that are not a multiple of 8: the first pass reads the scale
