@@ -589,8 +589,9 @@ template <bool RING> struct DxShape {
     static constexpr int P = RING ? 8 : 4;                   // pixels per lane, along a row
     static constexpr int H = 16 * P;                         // tile rows: 16 waves of P rows
     static constexpr int LH = 32 / P;                        // lanes per row in a half-wave
-    static constexpr int SLOT_ROWS = (H + 2 * 31 + 15) / 16;  // 16 + 16i rows cover H + 2R
-    static constexpr int SLOT_COLS = 2;                      // 64 + 64j columns cover 64 + 2R
+    static constexpr int SLOT_ROWS = (H + 2 * kMaxRadius + 15) / 16;  // rows r0 + 16 i cover H + 2R
+    static constexpr int SLOT_COLS = 2;  // columns c0 + 64 j cover 64 + 2R
+    static_assert(DXW + 2 * kMaxRadius <= 64 * SLOT_COLS, "the slots cover the widest window");
 };
 static_assert(DxShape<true>::H == kDxTileRowsRing && DxShape<false>::H == kDxTileRowsWeighted, "tile lists");
 
