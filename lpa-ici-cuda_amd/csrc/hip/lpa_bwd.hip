@@ -207,42 +207,47 @@ __device__ __forceinline__ void load_scales(const uint8_t *__restrict__ m, bool 
 // tap, against the tile walk's sliding window over all 512 pixels of the wave.
 // The tap offsets in the tile are read from tap_lds, one load per slice.
 // `pix_off` is the tile offset of the lane's first pixel from base0
-// (grad_lane_row * lp + grad_lane_col * GP): a pixel's offset is one readlane.
+// (grad_lane_row * lp + grad_lane_col * GP).  Offsets are taken in bytes,
+// the lanes' pixel offsets and the tap offsets once per call, so a pixel's
+// address is one readlane and each read one add.
 template <int IT>
 __device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int base0, int pix_off,
                                               const int *__restrict__ tap_lds, int w0, int taps, const float (&m)[GP],
                                               const int (&sd)[GP], int s, float *__restrict__ row, int lane)
 {
-    int o[IT];
+    // the lane's tap addresses, relative to the pixel offset added per read
+    const char *xo[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int t = 64 * i + lane;
-        o[i] = t < taps ? as_global(tap_lds)[w0 + t] : 0;
+        xo[i] = reinterpret_cast<const char *>(xt + (t < taps ? as_global(tap_lds)[w0 + t] : 0));
     }
+    const int pb = 4 * (base0 + pix_off);
     float acc[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) acc[i] = 0.0f;
     // lane L holds pixels grad_lane_col(L) * GP + p of row grad_lane_row(L) of the wave's block
     auto pixel = [&](int L, int p, float &c) {
         c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m[p]), L));
-        return base0 + __builtin_amdgcn_readlane(pix_off, L) + p;
+        return __builtin_amdgcn_readlane(pb, L) + 4 * p;
     };
+    auto at = [&](int b, int i) { return *reinterpret_cast<const float *>(xo[i] + b); };
 #pragma unroll
     for (int p = 0; p < GP; ++p) {
         unsigned long long mask = __ballot(sd[p] == s);
         // two pixels per step while two are left, so their reads overlap
-        while (mask & (mask - 1)) {
+        for (int n = __builtin_popcountll(mask); n >= 2; n -= 2) {
             const int L0 = __builtin_ctzll(mask);
-            mask &= mask - 1;
+            mask ^= 1ull << L0;
             const int L1 = __builtin_ctzll(mask);
-            mask &= mask - 1;
+            mask ^= 1ull << L1;
             float c0, c1;
             const int b0 = pixel(L0, p, c0), b1 = pixel(L1, p, c1);
             float v0[IT], v1[IT];
 #pragma unroll
             for (int i = 0; i < IT; ++i) {
-                v0[i] = xt[b0 + o[i]];
-                v1[i] = xt[b1 + o[i]];
+                v0[i] = at(b0, i);
+                v1[i] = at(b1, i);
             }
 #pragma unroll
             for (int i = 0; i < IT; ++i) acc[i] = __builtin_fmaf(c1, v1[i], __builtin_fmaf(c0, v0[i], acc[i]));
@@ -251,7 +256,7 @@ __device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int 
             float c0;
             const int b0 = pixel(__builtin_ctzll(mask), p, c0);
 #pragma unroll
-            for (int i = 0; i < IT; ++i) acc[i] = __builtin_fmaf(c0, xt[b0 + o[i]], acc[i]);
+            for (int i = 0; i < IT; ++i) acc[i] = __builtin_fmaf(c0, at(b0, i), acc[i]);
         }
     }
 #pragma unroll
