@@ -110,6 +110,31 @@ __device__ __forceinline__ float wave_total(float v)
     return row_total(c + d);
 }
 
+// Bit s: some pixel of the lane selected scale s (scale indices of pixels
+// outside the image are 255 and set no bit).
+__device__ __forceinline__ unsigned lane_scales(const int (&sd)[GP])
+{
+    unsigned b = 0;
+#pragma unroll
+    for (int p = 0; p < GP; ++p) b |= sd[p] < 32 ? 1u << sd[p] : 0u;
+    return b;
+}
+
+// OR of v over the 64 lanes of a wave, wave-uniform.
+__device__ __forceinline__ unsigned wave_or(unsigned v)
+{
+    float a = __uint_as_float(v), b = a;
+    swap32(a, b);
+    float c = __uint_as_float(__float_as_uint(a) | __float_as_uint(b)), d = c;
+    swap16(c, d);
+    unsigned u = __float_as_uint(c) | __float_as_uint(d);
+    u |= __float_as_uint(dpp<0xB1>(__uint_as_float(u)));   // quad_perm [1,0,3,2]
+    u |= __float_as_uint(dpp<0x4E>(__uint_as_float(u)));   // quad_perm [2,3,0,1]
+    u |= __float_as_uint(dpp<0x124>(__uint_as_float(u)));  // row_ror:4
+    u |= __float_as_uint(dpp<0x128>(__uint_as_float(u)));  // row_ror:8
+    return __builtin_amdgcn_readfirstlane(u);
+}
+
 // Sum L (<= 8) values over the 64 lanes of a wave and add the first n (<= L)
 // totals into row[0..n) with fire-and-forget float atomics.  Reduce-scatter: the swaps
 // halve the values per lane across lane bits 5 and 4 without any select, so
@@ -376,6 +401,8 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
             for (int d = 0; d < D; ++d) {
                 int sd[GP];
                 load_scales(img.map + (size_t)d * plane + pix, whole, inside, sd);
+                // scales selected by the lane's pixels, and by the wave's
+                const unsigned ls = lane_scales(sd), ws = wave_or(ls);
                 float yh[GP];
                 if (have_est) {
                     const auto ge = as_global(img.est) + (size_t)d * plane + pix;
@@ -394,10 +421,8 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                         rq[p] = 0u;
                     }
                     for (int s = 0; s < S; ++s) {
-                        bool need = false;
-#pragma unroll
-                        for (int p = 0; p < GP; ++p) need |= inside[p] && sd[p] >= s;
-                        if (!__any(need)) break;
+                        if ((ws >> s) == 0u) break;  // no pixel of the wave selected s or deeper
+                        const bool need = (ls >> s) != 0u;
                         const int k = d * S + s;
                         if (fixed) {
                             if (need) prefix_runs<GP>(lane_q, lpq, rings.runs, load_const(rings.kern + k), rq);
@@ -427,11 +452,8 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
                         const int s = s0 + j;
-                        bool want = false;
-#pragma unroll
-                        for (int p = 0; p < GP; ++p) want |= sd[p] == s;
                         float a = 0.0f;
-                        if (__any(want)) {
+                        if ((ws >> s) & 1u) {
 #pragma unroll
                             for (int p = 0; p < GP; ++p)
                                 if (sd[p] == s) a = __builtin_fmaf(q[p], dv[p], a);
@@ -452,11 +474,9 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
         for (int d = 0; d < D; ++d) {
             int sd[GP];
             load_scales(img.map + (size_t)d * plane + pix, whole, inside, sd);
+            const unsigned ws = wave_or(lane_scales(sd));  // scales selected in the wave
             for (int s = 0; s < S; ++s) {
-                bool want = false;
-#pragma unroll
-                for (int p = 0; p < GP; ++p) want |= sd[p] == s;
-                if (!__any(want)) continue;
+                if (!((ws >> s) & 1u)) continue;
                 const int k = d * S + s;
                 const float inv = inv_t[k];
                 float m[GP], yacc[GP];
