@@ -293,8 +293,9 @@ __device__ __forceinline__ void taps_on_lanes(const float *__restrict__ xt, int 
 
 // The cheaper of the two A walks for one kernel and wave, in measured time
 // units: the tile walk costs about kPieceCost per run piece whatever the
-// pixels; taps on lanes about kLanePixelCost per pixel and 64-tap slice.
-constexpr int kPieceCost = 100, kLanePixelCost = 12, kMaxLaneSlices = 4;
+// pixels; taps on lanes about kLanePixelCost per pixel and 64-tap slice (the
+// pass time is flat for lane costs from 6 to 12 on the shipped set).
+constexpr int kPieceCost = 100, kLanePixelCost = 8, kMaxLaneSlices = 4;
 
 // Pass 1.  For every tile: q = g / sum_d inv_d (written to the q map for pass
 // 2), then two phases.
