@@ -707,21 +707,20 @@ __global__ __launch_bounds__(DXT) void bwd_dx_kernel(DevKset ks, DevRings rings,
             for (int i = 0; i < SR; ++i)
 #pragma unroll
                 for (int j = 0; j < SC; ++j)
-                    sv[i][j] = (vb >> (i * SC + j)) & 1u ? (int)mp[off + (long long)16 * i * img.W + 64 * j] : 255;
+                    sv[i][j] = (vb >> (i * SC + j)) & 1u ? (int)mp[off + (long long)16 * i * img.W + 64 * j] : -1;
             if constexpr (RING) {
-                // The deepest scale selected in the window: the walk starts
-                // there, and every map from it down is non-empty.
+                // The deepest scale selected in the window (elements outside
+                // it hold -1): the walk starts there.
                 int deepest = -1;
 #pragma unroll
                 for (int i = 0; i < SR; ++i)
 #pragma unroll
-                    for (int j = 0; j < SC; ++j)
-                        if (sv[i][j] < S && qv[i][j] != 0.0f) deepest = max(deepest, sv[i][j]);
+                    for (int j = 0; j < SC; ++j) deepest = max(deepest, sv[i][j]);
                 const int top = block_max<DXT>(deepest);  // its first barrier: the last ring sums are done
                 if (top < 0) continue;
                 // Clear the map, then each step writes only the elements that
-                // join it; elements outside the window hold scale 255 and
-                // never join.
+                // join it; elements outside the window hold scale -1 and never
+                // join.
                 for (int e = tid; e < map4; e += DXT) mt4[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 for (int s = top; s >= 0; --s) {
                     __syncthreads();  // the clear, or the previous ring sums, done with the map
