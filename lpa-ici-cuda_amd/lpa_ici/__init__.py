@@ -246,6 +246,13 @@ def _stream_ptr(stream):
     return ctypes.c_void_p(s.cuda_stream)
 
 
+def same_layout(x):
+    """An uninitialised device tensor with x's shape and row pitch (outputs
+    y and dL/dx share the layout of the image x, whatever its pitch)."""
+    torch = _torch()
+    return torch.empty_strided(tuple(x.shape), x.stride(), dtype=x.dtype, device=x.device)
+
+
 def image_struct(x, y, scale_map=None, sigma=20.0 / 255.0, gamma=2.0, variances=None, estimates=None):
     """An lpa_image over device tensors x, y (H x W float32, row stride =
     pitch), an optional uint8 [D, H, W] scale map and optional float32
@@ -288,7 +295,7 @@ def denoise(ks, xs, sigmas=20.0 / 255.0, gammas=2.0, scale_maps=False, flags=0, 
     gam = gammas if isinstance(gammas, (list, tuple)) else [gammas] * n
     var = variances if isinstance(variances, (list, tuple)) else [variances] * n
     var = [None if v is None else _f32(v) for v in var]
-    ys = [torch.empty_like(x) for x in xs]
+    ys = [same_layout(x) for x in xs]
     maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device=x.device) for x in xs] \
         if scale_maps else [None] * n
     ests = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.float32, device=x.device) for x in xs] \
@@ -331,7 +338,7 @@ def backward(ks, xs, ys, maps, dys, sigmas=20.0 / 255.0, gammas=2.0, dkernel=Non
     n = len(xs)
     sig = sigmas if isinstance(sigmas, (list, tuple)) else [sigmas] * n
     gam = gammas if isinstance(gammas, (list, tuple)) else [gammas] * n
-    dxs = [torch.empty_like(x) for x in xs]
+    dxs = [same_layout(x) for x in xs]
     for d, y in zip(dys, ys):
         assert d.shape == y.shape and d.stride() == y.stride() and d.dtype == torch.float32
     if dkernel is None:

@@ -13,7 +13,7 @@ import os
 
 import torch
 
-from . import FWD_PLAIN, Plan, tile_configs
+from . import FWD_PLAIN, Plan, same_layout, tile_configs
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "lpa_ici", "autotune.json")
 
@@ -80,7 +80,7 @@ def autotune(ks, xs, flags=FWD_PLAIN, sigmas=20.0 / 255.0, gammas=2.0, cache=DEF
     if not force and key in table:
         ks.set_tile_config(int(table[key]["best"]))
         return int(table[key]["best"]), {}
-    ys = [torch.empty_like(x) for x in xs]
+    ys = [same_layout(x) for x in xs]
     times = {}
     for i in range(len(tile_configs())):
         times[i] = time_config(ks, i, xs, ys, flags, stream, reps=reps, sigmas=sigmas, gammas=gammas)

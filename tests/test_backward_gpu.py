@@ -270,3 +270,44 @@ def test_backward_needs_scale_map(ks):
     with pytest.raises(lpa_ici.LpaError) as e:
         lpa_ici.backward(ks, [xd], [y], [None], [torch.ones_like(y)])
     assert e.value.status == 1 and "scale map" in str(e.value)
+
+
+def test_adjoint_pitched_rows(ks):
+    """Images, upstream gradients and dL/dx with a row pitch wider than the
+    image (NaN in the inputs' padding, a sentinel in dL/dx's): dL/dx equals
+    the dense run bit for bit and its padding is never written; dL/dkernel
+    matches up to float reassociation."""
+    import ctypes
+    w, h, pitch = 133, 70, 160
+    x = noisy_synth(w, h, seed=45)
+
+    def view(a, fill):
+        buf = torch.full((h, pitch), fill, dtype=torch.float32, device="cuda")
+        if a is not None:
+            buf[:, :w] = a
+        return buf, buf[:, :w]
+
+    _, xv = view(torch.from_numpy(x).cuda(), float("nan"))
+    yv, m = lpa_ici.denoise(ks, xv, SIGMA, GAMMA, scale_maps=True)
+    g = torch.randn((h, w), generator=torch.Generator(device="cuda").manual_seed(9), device="cuda")
+    _, gv = view(g, float("nan"))
+    dx_buf, dxv = view(None, 7.0)
+    dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
+    arr = (lpa_ici.LpaImage * 1)(lpa_ici.image_struct(xv, yv, m, SIGMA, GAMMA))
+    lpa_ici.check(lpa_ici.lib().lpa_ici_backward(ks.handle, arr, (ctypes.c_void_p * 1)(gv.data_ptr()),
+                                                 (ctypes.c_void_p * 1)(dxv.data_ptr()),
+                                                 ctypes.c_void_p(dk.data_ptr()), 1,
+                                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    xd = torch.from_numpy(x).cuda()
+    y0, m0 = lpa_ici.denoise(ks, xd, SIGMA, GAMMA, scale_maps=True)
+    dx0, dk0 = lpa_ici.backward(ks, [xd], [y0], [m0], [g.contiguous()], SIGMA, GAMMA)
+    torch.cuda.synchronize()
+    assert torch.equal(m, m0)
+    assert torch.equal(dxv.contiguous().view(torch.int32), dx0[0].view(torch.int32))
+    assert bool((dx_buf[:, w:] == 7.0).all())
+    assert torch.allclose(dk, dk0, rtol=1e-5, atol=1e-6 * dk0.abs().max().item())
+    # the Python wrapper allocates dL/dx at the image's pitch
+    dxs, _ = lpa_ici.backward(ks, [xv], [yv], [m], [gv], SIGMA, GAMMA)
+    torch.cuda.synchronize()
+    assert dxs[0].stride() == xv.stride()
+    assert torch.equal(dxs[0].contiguous().view(torch.int32), dx0[0].view(torch.int32))

@@ -381,3 +381,28 @@ def test_full_size_windows_bitexact(ks):
         # pixels at least R inside the crop's own edges are compared
         assert np.array_equal(y[r0:r0 + n, c0:c0 + n].view(np.uint32), yo[inner].view(np.uint32)), (r0, c0)
         assert np.array_equal(maps[:, r0:r0 + n, c0:c0 + n], mo[(slice(None),) + inner]), (r0, c0)
+
+
+def pitched(a, pitch, fill=float("nan")):
+    """A device view of a (H x W) with row pitch `pitch` floats; the columns
+    past W hold `fill`."""
+    h, w = a.shape
+    buf = torch.full((h, pitch), fill, dtype=torch.float32, device="cuda")
+    buf[:, :w] = dev(a)
+    return buf[:, :w]
+
+
+@pytest.mark.parametrize("flags", [0, lpa_ici.FWD_EXACT, lpa_ici.FWD_PLAIN])
+def test_forward_pitched_rows(ks, flags):
+    """lpa_ici.denoise on rows with a pitch wider than the image (NaN in the
+    padding): the output is allocated at the input's pitch, and the result
+    is the dense image's, bit for bit, for three walks, so the padding is
+    never read."""
+    x = noisy_synth(150, 70, seed=44)
+    xv = pitched(x, 193)
+    y, m = lpa_ici.denoise(ks, xv, SIGMA, GAMMA, scale_maps=True, flags=flags)
+    y0, m0 = lpa_ici.denoise(ks, dev(x), SIGMA, GAMMA, scale_maps=True, flags=flags)
+    torch.cuda.synchronize()
+    assert y.stride() == xv.stride()
+    assert torch.equal(m, m0)
+    assert torch.equal(y.contiguous().view(torch.int32), y0.view(torch.int32))
