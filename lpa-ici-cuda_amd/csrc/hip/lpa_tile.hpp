@@ -340,8 +340,8 @@ __device__ __forceinline__ bool stage_prefix(uint32_t *__restrict__ q, int lp, c
 // by stage_prefix): a run (row dr, column c0, length L) adds
 // Q[dr][c0 + L + p] - Q[dr][c0 + p], whatever its length.  Runs go in pairs:
 // one scalar load brings both records, the four windows of P prefix values
-// are read together (4P reads in flight), and the pair costs about three
-// integer operations per pixel.  There is no branch on the run length.
+// are read together (4P reads in flight), and the pair costs four integer
+// operations per pixel.  There is no branch on the run length.
 template <int P>
 __device__ __forceinline__ void prefix_runs(const uint32_t *__restrict__ lane, int lp, const int4 *__restrict__ runs,
                                             int2 kr, uint32_t (&rs)[P])
