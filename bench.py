@@ -12,12 +12,13 @@ carries
   gaussian_m20  the same image with the 16x7 m=(2,0) gaussian-window set
   batched_uneven  256 images with sides U{128..2048}, per-image sigma/gamma:
                 plain (one launch per image) vs one persistent launch vs its
-                hipGraph replay
+                hipGraph replay, each form at its own autotuned tile
+                configuration
   fwd_bwd       64 x 512^2 forward + adjoint (dL/dx, dL/dkernel); the forward
                 saves the uint8 scale maps and the adjoint recomputes the
                 estimates from them; timed again with the forward keeping
                 each direction's selected estimate for the adjoint to read
-                ("kept_estimates")
+                ("kept_estimates"); the forward also one launch per image
   config5       32 images with sides U{1024..8192} (about 630 Mpx), fwd+bwd
                 in the same two forms, and each form's peak device memory in
                 use against the 288 GB of HBM
@@ -131,11 +132,18 @@ def batched_leg(ks, steps, warmup, stream, n_images=256, seed=3):
     ys_batch = [torch.empty_like(x) for x in xs]
     ys_graph = [torch.empty_like(x) for x in xs]
     out = {"images": n_images, "mpx": npx / 1e6}
-    plans = {
-        "plain": lpa_ici.Plan(ks, xs, ys_plain, None, sig, gam, flags=lpa_ici.FWD_PLAIN),
-        "batched": lpa_ici.Plan(ks, xs, ys_batch, None, sig, gam, flags=0),
-        "graph": lpa_ici.Plan(ks, xs, ys_graph, None, sig, gam, flags=lpa_ici.FWD_GRAPH),
-    }
+    # each form at the tile configuration the autotuner picks for it on this
+    # workload (a plan keeps the configuration it was built with)
+    from lpa_ici.autotune import autotune
+    saved = ks.tile_config()
+    plain_cfg, _ = autotune(ks, xs, flags=lpa_ici.FWD_PLAIN, sigmas=sig, gammas=gam, stream=stream)
+    plans = {"plain": lpa_ici.Plan(ks, xs, ys_plain, None, sig, gam, flags=lpa_ici.FWD_PLAIN)}
+    batch_cfg, _ = autotune(ks, xs, flags=0, sigmas=sig, gammas=gam, stream=stream)
+    plans["batched"] = lpa_ici.Plan(ks, xs, ys_batch, None, sig, gam, flags=0)
+    plans["graph"] = lpa_ici.Plan(ks, xs, ys_graph, None, sig, gam, flags=lpa_ici.FWD_GRAPH)
+    ks.set_tile_config(saved)
+    out["tile_config"] = {"plain": plain_cfg, "batched": batch_cfg}
+    log(f"[batched] tile configurations: plain {plain_cfg}, batched and graph {batch_cfg}")
     with torch.cuda.stream(stream):
         for name, plan in plans.items():
             ms = timed(lambda: plan.run(stream), steps, warmup, stream)
@@ -198,13 +206,27 @@ def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
     times = {v: adjoint_variant(ks, xs, ys, maps, SIGMA, GAMMA, dys, dxs, dk, steps, warmup, stream, v)
              for v in ("recompute", "kept_estimates")}
     npx = n * side * side
+    # the same forward, one launch per image, at the plain form's own tile
+    # configuration for these images
+    from lpa_ici.autotune import autotune
+    saved = ks.tile_config()
+    plain_cfg, _ = autotune(ks, xs, flags=lpa_ici.FWD_PLAIN, stream=stream)
+    plain = lpa_ici.Plan(ks, xs, ys, maps, SIGMA, GAMMA, flags=lpa_ici.FWD_PLAIN)
+    ks.set_tile_config(saved)
+    with torch.cuda.stream(stream):
+        ms_plain = timed(lambda: plain.run(stream), steps, warmup, stream)
+    plain.close()
+    log(f"[fwd+bwd] forward one launch per image: {ms_plain:.2f} ms (tile configuration {plain_cfg}), "
+        f"persistent {times['recompute'][0]:.2f} ms: {ms_plain / times['recompute'][0]:.2f}x")
     # Algorithmic adjoint work: per selected kernel tap of every pixel and
     # direction, 2 FLOP to recompute yhat, 2 for the dL/dx gather and 4 for
     # dL/dkernel (the reference's taps, whatever the walks execute).
     nnz = torch.from_numpy(ks.runs()["nnz"].astype(np.int64).reshape(ks.D, ks.S)).cuda()
     dsel = torch.arange(ks.D, device="cuda")[:, None, None]
     sel_taps = int(sum(nnz[dsel, m.long()].sum().item() for m in maps))
-    rec = {"images": n, "side": side, "selected_taps_per_px": sel_taps / npx}
+    rec = {"images": n, "side": side, "selected_taps_per_px": sel_taps / npx,
+           "fwd_plain_per_image_ms": ms_plain, "fwd_plain_tile_config": plain_cfg,
+           "fwd_batched_over_plain": ms_plain / times["recompute"][0]}
     for variant, (ms_f, ms_fb) in times.items():
         ms_b = ms_fb - ms_f
         bwd_tflops = 8.0 * sel_taps / (ms_b * 1e-3) / 1e12
