@@ -454,10 +454,7 @@ def main():
     del main_leg
     if not args.skip_batched:
         b = batched_leg(ks, max(3, args.steps // 4), 1, stream)
-        rec["batched_uneven"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in b.items()
-                                 if not isinstance(v, dict)}
-        for k in ("plain", "batched", "graph"):
-            rec["batched_uneven"][k] = {kk: round(vv, 3) for kk, vv in b[k].items()}
+        rec["batched_uneven"] = rounded(b)
     if not args.skip_fwdbwd:
         fb = fwd_bwd_leg(ks, max(3, args.steps // 4), 1, stream)
         rec["fwd_bwd"] = rounded(fb)
