@@ -18,7 +18,7 @@ __all__ = [
     "FWD_WEIGHTED", "FWD_RING_FLOAT",
     "LpaError", "KernelSet", "Plan", "load_kernels", "wedge_kernels", "write_kernels", "kernel_variances",
     "make_noisy", "psnr", "synth_image",
-    "load_image", "write_hdr", "denoise", "estimates", "backward", "image_struct", "tile_configs", "lib",
+    "load_image", "write_hdr", "denoise", "estimates", "backward", "image_struct", "same_layout", "tile_configs", "lib",
 ]
 
 
