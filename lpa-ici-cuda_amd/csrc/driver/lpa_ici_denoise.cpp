@@ -16,32 +16,32 @@
 
 int main(int argc, char **argv)
 {
-    const Config cfg = parse_args(argc, argv, "lpa_ici_denoise_gpu");
+    const lpa_cli::Settings cfg = lpa_cli::parse_command_line(argc, argv, "lpa_ici_denoise_gpu");
 
     int width = 0, height = 0;
     const int channels = cfg.channels;
     std::vector<float> img_gray;
-    if (lpa_load_image_channels(cfg.image_file.c_str(), channels, &width, &height, nullptr, 0) != LPA_OK) {
-        std::cerr << "Error in loading the image " << cfg.image_file << "\n";
+    if (lpa_load_image_channels(cfg.image.c_str(), channels, &width, &height, nullptr, 0) != LPA_OK) {
+        std::cerr << "Error in loading the image " << cfg.image << "\n";
         return 1;
     }
     img_gray.resize((size_t)width * height * channels);
-    if (lpa_load_image_channels(cfg.image_file.c_str(), channels, &width, &height, img_gray.data(),
+    if (lpa_load_image_channels(cfg.image.c_str(), channels, &width, &height, img_gray.data(),
                                 (long long)img_gray.size()) != LPA_OK) {
-        std::cerr << "Error in loading the image " << cfg.image_file << "\n";
+        std::cerr << "Error in loading the image " << cfg.image << "\n";
         return 1;
     }
     std::cout << "Loaded image with a width of " << width << " px, a height of " << height
               << " px and (imported) " << channels << " channels\n";
 
-    const std::string gray_file = sibling_path(cfg.output_file, "gray_image.hdr");
+    const std::string gray_file = lpa_cli::beside(cfg.output, "gray_image.hdr");
     std::cout << "Saving gray image as " << gray_file << "\n";
     lpa_write_hdr_channels(gray_file.c_str(), width, height, channels, img_gray.data());
 
     const int img_size = width * height * channels;
     std::vector<float> noisy(img_gray);
-    create_noisy_image(img_gray.data(), noisy.data(), img_size, cfg.sigma_noise, cfg.noise_seed);
-    const std::string noisy_file = sibling_path(cfg.output_file, "gray_image_noisy.hdr");
+    create_noisy_image(img_gray.data(), noisy.data(), img_size, cfg.sigma, cfg.seed);
+    const std::string noisy_file = lpa_cli::beside(cfg.output, "gray_image_noisy.hdr");
     std::cout << "Saving noisy gray image as " << noisy_file << "\n";
     lpa_write_hdr_channels(noisy_file.c_str(), width, height, channels, noisy.data());
 
@@ -51,35 +51,35 @@ int main(int argc, char **argv)
     int num_dirs = 0, num_scales = 0;
     std::vector<float> weights;
     std::vector<int> offsets, rows, cols;
-    if (!load_LPA_kernels_standardized(cfg.kernel_file, weights, offsets, rows, cols, num_dirs, num_scales))
+    if (!load_LPA_kernels_standardized(cfg.kernels, weights, offsets, rows, cols, num_dirs, num_scales))
         return 1;
     std::vector<float> variances((size_t)num_dirs * num_scales, 0.0f);
-    compute_kernel_variances(weights, offsets, rows, cols, variances, cfg.sigma_noise);
+    compute_kernel_variances(weights, offsets, rows, cols, variances, cfg.sigma);
 
     std::vector<float> denoised(img_size, 0.0f), wbuf(img_size, 0.0f);
     Timer t;
     if (channels == 1)
         anisotropic_lpa_ici_gpu_fused(noisy.data(), denoised.data(), width, height, channels, wbuf.data(), num_dirs,
                                       num_scales, weights.data(), (int)weights.size(), offsets.data(), rows.data(),
-                                      cols.data(), variances.data(), cfg.ici_gamma);
+                                      cols.data(), variances.data(), cfg.gamma);
     else
         lpa_ici_denoise_channels(noisy.data(), denoised.data(), width, height, channels, num_dirs, num_scales,
                                  weights.data(), offsets.data(), rows.data(), cols.data(), variances.data(),
-                                 cfg.ici_gamma);
+                                 cfg.gamma);
     const double seconds = t.elapsed();
     std::cout << "Time taken for lpa function execution: " << seconds << " seconds\n";
 
     const double psnr_denoised = calculate_PSNR(img_gray.data(), denoised.data(), img_size);
     std::cout << "Calculated PSNR of img_denoised: " << psnr_denoised << "\n";
-    std::cout << "Saving denoised image as " << cfg.output_file << "\n";
-    if (lpa_write_hdr_channels(cfg.output_file.c_str(), width, height, channels, denoised.data()) != LPA_OK)
+    std::cout << "Saving denoised image as " << cfg.output << "\n";
+    if (lpa_write_hdr_channels(cfg.output.c_str(), width, height, channels, denoised.data()) != LPA_OK)
         std::cerr << "Error: " << lpa_last_error() << "\n";
 
     if (!cfg.debug_dir.empty() && channels != 1)
         std::cerr << "--debug applies to one channel; skipped\n";
     else if (!cfg.debug_dir.empty())
         write_direction_outputs(noisy.data(), width, height, num_dirs, num_scales, weights.data(), offsets.data(),
-                                rows.data(), cols.data(), variances.data(), cfg.ici_gamma, cfg.debug_dir);
+                                rows.data(), cols.data(), variances.data(), cfg.gamma, cfg.debug_dir);
 
     std::printf("{\"width\": %d, \"height\": %d, \"psnr_noisy\": %.10f, \"psnr_denoised\": %.10f, "
                 "\"seconds\": %.6f}\n",

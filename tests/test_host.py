@@ -321,6 +321,48 @@ def test_driver_help_and_errors(tmp_path):
     assert "Calculated PSNR of host_img_gray_noisy: 22.1245" in r.stdout
 
 
+def test_driver_usage_and_echo_text(tmp_path):
+    """The usage lines the reference prints (src/Config.hpp:27-37), column for
+    column, with the two added flags among them; the echo block and the
+    default output name (src/Config.hpp:79-89)."""
+    r = driver("--help")
+    assert r.returncode == 0
+    lines = r.stdout.splitlines()
+    assert lines[0] == f"Usage: {DRIVER} [options]"
+    assert lines[1:] == [
+        "  -i, --image <file>     input image (default data/barbara.png)",
+        "  -k, --kernels <file>   LPA kernel file (default "
+        "kernels/lpa_kernels_m_1_0_d_16_h12_1_2_4_8_16_24_32_symmetric.txt)",
+        "  -o, --output <file>    denoised .hdr output (default <image dir>/<image name>_<version>.hdr)",
+        "  -s, --sigma <value>    noise standard deviation on the 0-255 scale (default 20)",
+        "  -g, --gamma <value>    ICI threshold (default 2)",
+        "      --seed <value>     noise seed (default 20250910)",
+        "      --debug <dir>      also write each direction's estimate and scale map as .hdr in <dir>",
+        "  -c, --channels <1|3>   1: gray (default); 3: RGB, every channel denoised on its own",
+        "  -h, --help             show this message",
+    ]
+    # an unknown flag without a value reads as a missing value; an error is
+    # "Error: ..." and a blank line on stderr, the usage on stdout
+    r = driver("--bogus")
+    assert r.returncode == 1 and r.stderr.startswith("Error: unknown option or missing value for '--bogus'\n\n")
+    assert r.stdout.splitlines()[0] == f"Usage: {DRIVER} [options]"
+    r = driver("-c", "2")
+    assert r.returncode == 1 and "-c needs 1 or 3, got '2'" in r.stderr
+    r = driver("-s", "0")
+    assert r.returncode == 1 and "-s needs a positive number, got '0'" in r.stderr
+    # a value that looks like a flag is a value; the output name keeps every
+    # dot of the stem but the last
+    img = str(tmp_path / "a.b.png")
+    r = driver("--image", img, "-s", "10", "-g", "1.5", "--seed", "7", "-k", "-h")
+    assert r.returncode == 1 and "Error in loading the image" in r.stderr
+    assert r.stdout.splitlines()[:4] == [f"Image: {img}", "Kernels: -h", f"Output: {tmp_path}/a.b_lpa_ici_denoise_gpu.hdr",
+                                         "Sigma: 10/255, gamma: 1.5, seed: 7"]
+    r = driver("-i", "noext", "-o", str(tmp_path / "o.hdr"))
+    assert f"Output: {tmp_path}/o.hdr" in r.stdout
+    r = driver("-i", "noext")
+    assert "Output: noext_lpa_ici_denoise_gpu.hdr" in r.stdout
+
+
 def write_png(path, img):
     """8-bit PNG (gray [H, W] or RGB [H, W, 3]), filter 0, via zlib."""
     import struct
