@@ -40,10 +40,11 @@ def _lib():
         lib.oracle_walk_f64.argtypes = common + [_D, _D, _P, _P, _P, _I]
         lib.oracle_walk_q.argtypes = common + [_P, _F, _I, _F, _P, _P, _I]
         lib.oracle_adjoint_f64.argtypes = [_P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _D, _P, _P, _P]
+        lib.oracle_adjoint_f64_planted.argtypes = [_P, _P, _I, _I, _I, _I, _P, _P, _P, _P, _D, _P, _P, _P, _I]
         lib.oracle_useful_taps.argtypes = [_P, ctypes.c_size_t, _I, _I, _P]
         lib.oracle_useful_taps.restype = ctypes.c_longlong
         for f in ("oracle_v0_0", "oracle_v1_0", "oracle_walk_f32", "oracle_walk_q", "oracle_walk_f64",
-                  "oracle_adjoint_f64"):
+                  "oracle_adjoint_f64", "oracle_adjoint_f64_planted"):
             getattr(lib, f).restype = _I
         _oracle = lib
     return _oracle
@@ -129,15 +130,19 @@ def walk_f64(x, ks, sigma, gamma, frozen=None, threads=None, want_est=False):
     return (y, maps, est) if want_est else (y, maps)
 
 
-def adjoint_f64(x, dy, ks, sigma, scales):
-    """FP64 dL/dx (H x W) and dL/dk (dense layout, like ks.weights)."""
+def adjoint_f64(x, dy, ks, sigma, scales, planted_mistake=0):
+    """FP64 dL/dx (H x W) and dL/dk (dense layout, like ks.weights).
+    planted_mistake 1..5 selects one of the oracle's planted mistakes (for the
+    test that shows the finite-difference pin catches them)."""
     x, W, H = _img(x)
     dy = np.ascontiguousarray(dy, np.float32)
     sc = np.ascontiguousarray(scales, np.uint8)
     dx = np.zeros((H, W), np.float64)
     dk = np.zeros(ks.weights.size, np.float64)
-    if _lib().oracle_adjoint_f64(_ptr(x), _ptr(dy), W, H, ks.D, ks.S, *_kargs(ks), float(sigma), _ptr(sc), _ptr(dx),
-                                 _ptr(dk)):
+    args = (_ptr(x), _ptr(dy), W, H, ks.D, ks.S, *_kargs(ks), float(sigma), _ptr(sc), _ptr(dx), _ptr(dk))
+    rc = (_lib().oracle_adjoint_f64_planted(*args, int(planted_mistake)) if planted_mistake
+          else _lib().oracle_adjoint_f64(*args))
+    if rc:
         raise MemoryError
     return dx, dk
 

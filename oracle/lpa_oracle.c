@@ -34,10 +34,22 @@
  *                      the sum in exact integers).  Pinned by the closed form
  *                      of a constant image, an impulse (orientation), the
  *                      f32 walk's tolerance bar and the FP64 walk.
- *   oracle_adjoint_f64 dL/dx and dL/dk of the frozen-selection forward.
- *                      Pinned by central finite differences of
- *                      oracle_walk_f64 with frozen scales (the output is
- *                      linear in x and rational in k).
+ *   oracle_adjoint_f64 dL/dx and dL/dk of the frozen-selection forward (no
+ *                      reference counterpart: the reference has no adjoint).
+ *                      Pinned on its own, on the CPU, by central finite
+ *                      differences of oracle_walk_f64 with frozen scales
+ *                      (tests/test_oracle.py::test_adjoint_f64_central_differences):
+ *                      dL/dx at 48 pixels (two-point differences, exact for a
+ *                      function linear in x) and dL/dk at 48 dense taps (the
+ *                      fourth-order difference; the output is rational in k
+ *                      through sum w^2), on random frozen scale maps, for the
+ *                      uniform symmetric set, the m=(2,0) gaussian set
+ *                      (non-uniform, negative taps) and two square sets.
+ *                      oracle_adjoint_f64_planted carries five plausible
+ *                      mistakes the same test must catch (a wrong tap's weight
+ *                      in the variance term, a rotated kernel in dL/dx, the
+ *                      variance term's factor 2 dropped, y dropped from
+ *                      yhat - y, border taps read as edge copies).
  */
 #include <math.h>
 #include <stdint.h>
@@ -482,10 +494,10 @@ int oracle_walk_f64(const float* noisy, double* out, int W, int H, int D, int S,
  * Taps that fall outside the image read zero.  dk is laid out like the dense
  * weights (offsets/rows/cols) and is accumulated into (caller zero-fills).
  * dx is overwritten. */
-int oracle_adjoint_f64(const float* noisy, const float* dy, int W, int H, int D, int S,
+static int adjoint_f64(const float* noisy, const float* dy, int W, int H, int D, int S,
                        const float* weights, const int* offsets, const int* rows,
                        const int* cols, double sigma, const uint8_t* scales,
-                       double* dx, double* dk)
+                       double* dx, double* dk, int mistake)
 {
     int K = D * S;
     double* ss = (double*)malloc(sizeof(double) * K);
@@ -523,23 +535,48 @@ int oracle_adjoint_f64(const float* noisy, const float* dy, int W, int H, int D,
                 const float* w = weights + offsets[k];
                 int kr = rows[k], kc = cols[k];
                 double cd = g * wdir[d] / Wt;
-                double corr = 2.0 * (yhat[d] - y) / ss[k];
+                double corr = (mistake == 3 ? 1.0 : 2.0) * (yhat[d] - (mistake == 4 ? 0.0 : y)) / ss[k];
                 double* gk = dk + offsets[k];
                 for (int a = 0; a < kr; a++)
                     for (int b = 0; b < kc; b++) {
                         int ir = r - kr / 2 + a, ic = c - kc / 2 + b;
                         double xv = 0.0;
                         int inside = (ir >= 0 && ir < H && ic >= 0 && ic < W);
-                        if (inside) xv = (double)noisy[(size_t)ir * W + ic];
+                        if (inside) {
+                            xv = (double)noisy[(size_t)ir * W + ic];
+                        } else if (mistake == 5) {
+                            int er = ir < 0 ? 0 : ir >= H ? H - 1 : ir, ec = ic < 0 ? 0 : ic >= W ? W - 1 : ic;
+                            xv = (double)noisy[(size_t)er * W + ec];
+                        }
                         double kv = (double)w[a * kc + b];
-                        gk[a * kc + b] += cd * (xv - kv * corr);
-                        if (inside) dx[(size_t)ir * W + ic] += cd * kv;
+                        double kvar = mistake == 1 ? (double)w[(a * kc + b + 1) % (kr * kc)] : kv;
+                        double kdx = mistake == 2 ? (double)w[(kr - 1 - a) * kc + (kc - 1 - b)] : kv;
+                        gk[a * kc + b] += cd * (xv - kvar * corr);
+                        if (inside) dx[(size_t)ir * W + ic] += cd * kdx;
                     }
             }
         }
     }
     free(yhat); free(wdir); free(ss);
     return 0;
+}
+
+int oracle_adjoint_f64(const float* noisy, const float* dy, int W, int H, int D, int S,
+                       const float* weights, const int* offsets, const int* rows,
+                       const int* cols, double sigma, const uint8_t* scales,
+                       double* dx, double* dk)
+{
+    return adjoint_f64(noisy, dy, W, H, D, S, weights, offsets, rows, cols, sigma, scales, dx, dk, 0);
+}
+
+/* The same with a planted mistake (1..5, see the header), for the test that
+ * shows the finite-difference check catches each; 0 is the adjoint itself. */
+int oracle_adjoint_f64_planted(const float* noisy, const float* dy, int W, int H, int D, int S,
+                               const float* weights, const int* offsets, const int* rows,
+                               const int* cols, double sigma, const uint8_t* scales,
+                               double* dx, double* dk, int mistake)
+{
+    return adjoint_f64(noisy, dy, W, H, D, S, weights, offsets, rows, cols, sigma, scales, dx, dk, mistake);
 }
 
 /* Useful taps of the walk: sum over pixels and directions of the nonzero

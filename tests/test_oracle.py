@@ -371,3 +371,108 @@ def test_walk_q_refuses_out_of_range(ks):
     x[3, 3] = np.nan
     with pytest.raises(ValueError):
         oracle.walk_q(x, ks, ks.variances(SIGMA), GAMMA, F, rng)
+
+
+# ---------------------------------------------------------------------------
+# oracle_adjoint_f64, pinned on its own by central differences of the FP64
+# walk with frozen scales
+# ---------------------------------------------------------------------------
+
+FD_SETS = [SYMMETRIC, GAUSSIAN_M20, "lpa_kernels_m_1_0_d_4_square.txt", "lpa_kernels_m_0_0_d_4_square.txt"]
+FD_EPS = 1e-3
+
+
+class _Tapped:
+    """A kernel set with one dense weight replaced (the FD abscissa)."""
+
+    def __init__(self, ks, idx, value):
+        self.D, self.S, self.offsets, self.rows, self.cols = ks.D, ks.S, ks.offsets, ks.rows, ks.cols
+        self.weights = ks.weights.copy()
+        self.weights[idx] = np.float32(value)
+
+
+def _fd_reference(ks, seed):
+    """FP64 central differences of L = sum(g * y) at 48 pixels (two-point:
+    y is linear in x once the scales are frozen) and 48 dense taps (fourth
+    order: y is rational in the taps through sum w^2), on a 26 x 22 image with
+    random frozen scales, so every kernel, large ones at the borders
+    included, is differentiated."""
+    rng = np.random.default_rng(seed)
+    H, W = 22, 26
+    x = noisy_synth(W, H, seed=seed)
+    maps = rng.integers(0, ks.S, size=(ks.D, H, W)).astype(np.uint8)
+    g = rng.standard_normal((H, W)).astype(np.float32)
+    g64 = g.astype(np.float64)
+
+    def loss(xx, kk):
+        y, _ = oracle.walk_f64(xx, kk, SIGMA, GAMMA, frozen=maps, threads=1)
+        return float((g64 * y).sum())
+
+    pix = [(0, 0), (H - 1, W - 1), (0, W - 1)] + [(int(rng.integers(0, H)), int(rng.integers(0, W)))
+                                                 for _ in range(45)]
+    fd_x = []
+    for r, c in pix:
+        xp, xm = x.copy(), x.copy()
+        xp[r, c] += FD_EPS
+        xm[r, c] -= FD_EPS
+        step = float(np.float64(xp[r, c]) - np.float64(xm[r, c]))
+        fd_x.append((loss(xp, ks) - loss(xm, ks)) / step)
+    # taps: 32 nonzero and 16 zero ones inside the kernels' bounding boxes
+    nz = np.flatnonzero(ks.weights != 0)
+    z = np.flatnonzero(ks.weights == 0)
+    taps = list(rng.choice(nz, 32, replace=False)) + (list(rng.choice(z, 16, replace=False)) if z.size >= 16 else [])
+    fd_k = []
+    for t in taps:
+        w0 = ks.weights[t]
+        hs = (-2, -1, 1, 2)
+        sets = [_Tapped(ks, t, np.float64(w0) + h * FD_EPS) for h in hs]
+        xs = np.array([np.float64(s.weights[t]) - np.float64(w0) for s in sets])
+        ys = np.array([loss(x, s) for s in sets])
+        fd_k.append(float(np.polyfit(xs, ys, 3)[2]))
+    return x, g, maps, pix, np.array(fd_x), np.array(taps), np.array(fd_k)
+
+
+def _fd_errors(ks, ref, mistake=0):
+    x, g, maps, pix, fd_x, taps, fd_k = ref
+    dx, dk = oracle.adjoint_f64(x, g, ks, SIGMA, maps, planted_mistake=mistake)
+    ex = max(abs(dx[r, c] - f) for (r, c), f in zip(pix, fd_x)) / np.abs(fd_x).max()
+    ek = np.abs(dk[taps] - fd_k).max() / np.abs(fd_k).max()
+    return ex, ek
+
+
+_fd_cache = {}
+
+
+def _fd_for(name):
+    if name not in _fd_cache:
+        _fd_cache[name] = _fd_reference(kernels(name), seed=len(name))
+    return _fd_cache[name]
+
+
+@pytest.mark.parametrize("name", FD_SETS)
+def test_adjoint_f64_central_differences(name):
+    """dL/dx within 1e-9 and dL/dk within 1e-6 of FP64 central differences,
+    relative to each gradient's largest component checked."""
+    ks = kernels(name)
+    ex, ek = _fd_errors(ks, _fd_for(name))
+    print(f"{name}: adjoint_f64 vs central differences: dx {ex:.1e}, dk {ek:.1e}")
+    assert ex < 1e-9 and ek < 1e-6
+
+
+def test_adjoint_f64_planted_mistakes_are_caught():
+    """Each planted mistake fails the finite-difference pin on at least one
+    set, by far more than its tolerance: a wrong tap's weight in the
+    variance term (invisible on uniform sets, caught on the gaussian set), a
+    rotated kernel in dL/dx, the variance term's factor 2 dropped, y dropped
+    from yhat - y, taps outside the image read as edge copies."""
+    caught = {}
+    for name in FD_SETS:
+        ks = kernels(name)
+        ref = _fd_for(name)
+        for m in range(1, 6):
+            ex, ek = _fd_errors(ks, ref, m)
+            if ex > 1e-3 or ek > 1e-3:
+                caught.setdefault(m, []).append(name)
+    print({m: len(v) for m, v in caught.items()})
+    assert sorted(caught) == [1, 2, 3, 4, 5], caught
+    assert GAUSSIAN_M20 in caught[1]
