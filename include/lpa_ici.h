@@ -197,6 +197,17 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image* imgs, float* const*
 lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image* imgs, const float* const* dy, float* const* dx,
                             float* dkernel, int n, void* stream);
 
+/* Peak device memory the library's per-call workspaces (lpa_ici_forward,
+ * lpa_ici_estimates, lpa_ici_backward: descriptors, tables, tile queues, the
+ * adjoint's q map and partial rows) have held at once since the last reset,
+ * in bytes: the used-memory high-water mark of the current device's default
+ * stream-ordered pool, which those workspaces come from.  reset != 0 clears
+ * the mark after reading it.  No counterpart in the reference (which
+ * allocates with cudaMalloc per call, src/v4_1_lpa_ici_2D_sparse_shared.cu:
+ * 1142-1157); it lets a caller state peak HBM use, inputs and outputs
+ * (caller-owned) added.  Errors: LPA_EINVAL, LPA_EHIP. */
+lpa_status lpa_workspace_peak(unsigned long long* bytes, int reset);
+
 /* ---------------------------------------------------------------------- */
 /* Host substrate of the driver (all host memory, synchronous)              */
 /* ---------------------------------------------------------------------- */

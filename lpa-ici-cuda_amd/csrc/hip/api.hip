@@ -728,6 +728,24 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     return LPA_OK;
 }
 
+lpa_status lpa_workspace_peak(unsigned long long *bytes, int reset)
+{
+    if (!bytes) return fail(LPA_EINVAL, "null output");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipMemPool_t pool = nullptr;
+    HIP_TRY(hipDeviceGetDefaultMemPool(&pool, dev));
+    uint64_t high = 0;
+    HIP_TRY(hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemHigh, &high));
+    *bytes = (unsigned long long)high;
+    if (reset) {
+        uint64_t zero = 0;
+        HIP_TRY(hipMemPoolSetAttribute(pool, hipMemPoolAttrUsedMemHigh, &zero));
+    }
+    lpa::clear_error();
+    return LPA_OK;
+}
+
 lpa_status lpa_plan_create(lpa_kset_t ks, const lpa_image *imgs, int n, unsigned flags, lpa_plan_t *out)
 {
     if (!out) return fail(LPA_EINVAL, "null output handle");

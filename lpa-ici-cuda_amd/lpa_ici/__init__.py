@@ -19,6 +19,7 @@ __all__ = [
     "LpaError", "KernelSet", "Plan", "load_kernels", "wedge_kernels", "write_kernels", "kernel_variances",
     "make_noisy", "psnr", "synth_image",
     "load_image", "write_hdr", "denoise", "estimates", "backward", "image_struct", "same_layout", "tile_configs", "lib",
+    "workspace_peak",
 ]
 
 
@@ -220,6 +221,14 @@ def write_hdr(path, img):
         check(lib().lpa_write_hdr(path.encode(), img.shape[1], img.shape[0], _ptr(img)))
     else:
         check(lib().lpa_write_hdr_channels(path.encode(), img.shape[1], img.shape[0], img.shape[2], _ptr(img)))
+
+
+def workspace_peak(reset=False):
+    """Bytes of device memory the library's per-call workspaces held at once
+    since the last reset (lpa_workspace_peak)."""
+    v = ctypes.c_ulonglong()
+    check(lib().lpa_workspace_peak(ctypes.byref(v), int(bool(reset))))
+    return v.value
 
 
 def tile_configs():

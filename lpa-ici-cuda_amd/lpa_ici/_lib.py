@@ -79,6 +79,7 @@ SIGNATURES = {
     "lpa_kset_get_tile_config": (_I, [_P]),
     "lpa_ici_estimates": (_I, [_P, ctypes.POINTER(LpaImage), _P, _I, ctypes.c_uint, _P]),
     "lpa_ici_backward": (_I, [_P, ctypes.POINTER(LpaImage), _P, _P, _P, _I, _P]),
+    "lpa_workspace_peak": (_I, [ctypes.POINTER(ctypes.c_ulonglong), _I]),
     "lpa_load_kernels": (_I, [ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                               _P, _P, _P, _P]),
     "lpa_kernel_variances": (None, [_I, _P, _P, _P, _P, _F, _P]),

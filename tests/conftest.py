@@ -50,3 +50,13 @@ def noisy_synth(w, h, seed=1, sigma=SIGMA, noise_seed=SEED):
 
 def golden(name):
     return np.load(os.path.join(GOLDEN, name))
+
+
+def dense_index(ks, r, i):
+    """(kernel, dense weight index) of run weight i; r = ks.runs()."""
+    run = int(np.searchsorted(r["run_woff"], i, side="right") - 1)
+    k = int(np.searchsorted(r["kernel_first_run"], run, side="right") - 1)
+    R, C = ks.rows[k], ks.cols[k]
+    row = r["run_row"][run] + R // 2
+    col = r["run_col"][run] + C // 2 + (i - r["run_woff"][run])
+    return k, int(ks.offsets[k] + row * C + col)

@@ -20,7 +20,7 @@ import torch
 
 import lpa_ici
 import oracle
-from conftest import GAMMA, GAUSSIAN_M20, SIGMA, kernels, noisy_synth
+from conftest import GAMMA, GAUSSIAN_M20, SIGMA, dense_index, kernels, noisy_synth
 from lpa_ici.autograd import lpa_ici as lpa_ici_fn, template_taps
 
 pytestmark = pytest.mark.gpu
@@ -32,17 +32,6 @@ def forward_gpu(ks, x, sigma=SIGMA):
     xd = torch.from_numpy(x).cuda()
     y, m = lpa_ici.denoise(ks, xd, sigma, GAMMA, scale_maps=True)
     return xd, y, m
-
-
-def dense_index(ks, r, i):
-    """Dense weight index of run weight i."""
-    k = int(np.searchsorted(r["kernel_first_run"], np.searchsorted(r["run_woff"], i, side="right") - 1,
-                            side="right") - 1)
-    run = int(np.searchsorted(r["run_woff"], i, side="right") - 1)
-    R, C = ks.rows[k], ks.cols[k]
-    row = r["run_row"][run] + R // 2
-    col = r["run_col"][run] + C // 2 + (i - r["run_woff"][run])
-    return k, int(ks.offsets[k] + row * C + col)
 
 
 class Modified:
