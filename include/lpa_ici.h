@@ -133,6 +133,26 @@ enum {
  * height above 65535, or more than 2^31 pixels), LPA_ENOMEM, LPA_EHIP. */
 lpa_status lpa_ici_forward(lpa_kset_t ks, const lpa_image* imgs, int n, unsigned flags, void* stream);
 
+/* lpa_ici_forward with no device allocation at all.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as lpa_ici_forward; the
+ * per-call workspace (work counter, image descriptors, per-image tables and
+ * the tile queue) lives in a static 4 MiB array that each device holds until
+ * the process ends, instead of a stream-ordered allocation.
+ * Limit: 256 + 56 n + 12 K n + 8 T bytes (each term rounded up to 256; K =
+ * D*S kernels, T = tiles of the set's tile configuration over the batch,
+ * 64x64 px by default) must fit in 4 MiB: for instance one 32768 x 32768
+ * image, or about 2,900 small images of the default set; beyond that
+ * LPA_ETOOBIG.  The kernel set must already be resident on the current
+ * device (its first lpa_ici_forward, lpa_plan_create, lpa_ici_estimates or
+ * lpa_ici_backward call makes it so); otherwise LPA_EINVAL.  Every call on
+ * one device shares the array, which keeps the last call's contents: calls
+ * on one stream are ordered, calls that overlap on one device (other
+ * streams, other threads) race.  Like lpa_ici_forward it copies the tables
+ * from host memory and waits for that copy. */
+lpa_status lpa_ici_forward_noalloc(lpa_kset_t ks, const lpa_image* imgs, int n, unsigned flags, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Plans: descriptors, tables and work queue uploaded once, replayed often  */
 /* ---------------------------------------------------------------------- */

@@ -17,6 +17,15 @@
 
 using lpa::fail;
 
+// Workspace of lpa_ici_forward_noalloc: the work counter, image descriptors,
+// per-image tables and tile queue of one call, in a fixed array that every
+// device holds for the life of the process.  Host code must never pass the
+// array's name to a kernel or a copy (on the host it is only a stand-in for
+// the symbol): it uses the device address hipGetSymbolAddress returns, looked
+// up once per device (scratch_address).
+constexpr size_t kStaticWorkspaceBytes = 4u << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticWorkspaceBytes];
+
 struct lpa_kset {
     int D = 0, S = 0, K = 0;
     int radius = 0;
@@ -374,8 +383,18 @@ struct lpa_plan {
 
 namespace {
 
+// Where a plan's device state lives: a stream-ordered allocation freed after
+// the launch (lpa_ici_forward), an allocation the plan owns (lpa_plan_create),
+// or a caller-provided buffer of `cap` bytes that nothing allocates or frees
+// (lpa_ici_forward_noalloc).
+struct PlanMemory {
+    enum Kind { kAsync, kOwned, kBuffer } kind;
+    void *buffer;
+    size_t cap;
+};
+
 // Lay out one plan allocation: [counter | descriptors | tables | tiles].
-lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags, bool async, hipStream_t stream,
+lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags, PlanMemory mem, hipStream_t stream,
                       lpa_plan *p)
 {
     p->ks = ks;
@@ -394,13 +413,22 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
     const size_t off_tab = off_desc + ((sizeof(lpa::ImgDesc) * n + 255) / 256) * 256;
     const size_t off_tiles = off_tab + ((sizeof(float) * 3 * K * n + 255) / 256) * 256;
     const size_t bytes = off_tiles + sizeof(uint2) * tiles.size();
-    std::vector<unsigned char> host(bytes, 0);
-    if (async) {
-        HIP_TRY(hipMallocAsync(&p->d_mem, bytes, stream));
+    unsigned char *base = nullptr;
+    if (mem.kind == PlanMemory::kBuffer) {
+        if (bytes > mem.cap)
+            return fail(LPA_ETOOBIG, "the batch needs " + std::to_string(bytes) + " bytes of workspace (descriptors, " +
+                                         "tables, tile queue); the static workspace holds " +
+                                         std::to_string(mem.cap));
+        base = (unsigned char *)mem.buffer;
     } else {
-        HIP_TRY(hipMalloc(&p->d_mem, bytes));
+        if (mem.kind == PlanMemory::kAsync) {
+            HIP_TRY(hipMallocAsync(&p->d_mem, bytes, stream));
+        } else {
+            HIP_TRY(hipMalloc(&p->d_mem, bytes));
+        }
+        base = (unsigned char *)p->d_mem;
     }
-    unsigned char *base = (unsigned char *)p->d_mem;
+    std::vector<unsigned char> host(bytes, 0);
     p->d_counter = (int *)base;
     p->d_desc = (lpa::ImgDesc *)(base + off_desc);
     p->d_tiles = (uint2 *)(base + off_tiles);
@@ -423,10 +451,32 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
     }
     std::memcpy(host.data() + off_desc, p->host_desc.data(), sizeof(lpa::ImgDesc) * n);
     std::memcpy(host.data() + off_tiles, tiles.data(), sizeof(uint2) * tiles.size());
-    HIP_TRY(hipMemcpyAsync(p->d_mem, host.data(), bytes, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(base, host.data(), bytes, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));  // `host` goes out of scope on return
     const int bpc = lpa::blocks_per_cu(p->shape, ks->radius, ks->K);
     p->grid = std::max(1, std::min(p->num_tiles, num_cus() * bpc));
+    return LPA_OK;
+}
+
+// Device address of `scratch` on the current device, looked up on the first
+// call per device and kept (each device holds its own copy of the array).
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+void *scratch_ptr[kMaxDevices] = {};
+
+lpa_status scratch_address(void **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(LPA_EINVAL, "device index " + std::to_string(dev) + " out of range");
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(LPA_EHIP, "hipGetSymbolAddress returned a null address for the static workspace");
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
     return LPA_OK;
 }
 
@@ -571,12 +621,40 @@ lpa_status lpa_ici_forward(lpa_kset_t ks, const lpa_image *imgs, int n, unsigned
     if (flags & LPA_FWD_GRAPH) return fail(LPA_EINVAL, "LPA_FWD_GRAPH applies to plans only");
     hipStream_t s = (hipStream_t)stream;
     lpa_plan p;
-    st = plan_build(ks, imgs, n, flags, true, s, &p);
+    st = plan_build(ks, imgs, n, flags, PlanMemory{PlanMemory::kAsync, nullptr, 0}, s, &p);
     if (st == LPA_OK) st = plan_launch(&p, s);
     if (p.d_mem) {
         hipError_t e = hipFreeAsync(p.d_mem, s);
         if (st == LPA_OK && e != hipSuccess) st = hip_fail(e, "hipFreeAsync");
     }
+    if (st == LPA_OK) lpa::clear_error();
+    return st;
+}
+
+lpa_status lpa_ici_forward_noalloc(lpa_kset_t ks, const lpa_image *imgs, int n, unsigned flags, void *stream)
+{
+    lpa_status st = check_images(ks, imgs, n);
+    if (st != LPA_OK) return st;
+    if (flags & LPA_FWD_GRAPH) return fail(LPA_EINVAL, "LPA_FWD_GRAPH applies to plans only");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    {
+        // The set's device tables are allocated on its first use; this entry
+        // point allocates nothing, so the set must be resident already.
+        std::lock_guard<std::mutex> lock(ks->upload_mutex);
+        if (!ks->uploaded)
+            return fail(LPA_EINVAL, "the kernel set is not resident on a device yet (its first lpa_ici_forward, "
+                                    "lpa_plan_create, lpa_ici_estimates or lpa_ici_backward call uploads it)");
+        if (ks->device != dev)
+            return fail(LPA_EINVAL, "the kernel set is resident on device " + std::to_string(ks->device) +
+                                        ", not on the current device " + std::to_string(dev));
+    }
+    void *buf = nullptr;
+    if ((st = scratch_address(&buf)) != LPA_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    lpa_plan p;
+    st = plan_build(ks, imgs, n, flags, PlanMemory{PlanMemory::kBuffer, buf, kStaticWorkspaceBytes}, s, &p);
+    if (st == LPA_OK) st = plan_launch(&p, s);
     if (st == LPA_OK) lpa::clear_error();
     return st;
 }
@@ -755,7 +833,7 @@ lpa_status lpa_plan_create(lpa_kset_t ks, const lpa_image *imgs, int n, unsigned
     if ((st = ensure_uploaded(ks)) != LPA_OK) return st;
     lpa_plan *p = new (std::nothrow) lpa_plan;
     if (!p) return fail(LPA_ENOMEM, "out of host memory");
-    st = plan_build(ks, imgs, n, flags, false, nullptr, p);
+    st = plan_build(ks, imgs, n, flags, PlanMemory{PlanMemory::kOwned, nullptr, 0}, nullptr, p);
     if (st != LPA_OK) {
         lpa_plan_destroy(p);
         return st;

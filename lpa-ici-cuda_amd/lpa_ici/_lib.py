@@ -70,6 +70,7 @@ SIGNATURES = {
     "lpa_kset_get_info": (_I, [_P, ctypes.POINTER(KsetInfo)]),
     "lpa_kset_get_runs": (_I, [_P] + [_P] * 9),
     "lpa_ici_forward": (_I, [_P, ctypes.POINTER(LpaImage), _I, ctypes.c_uint, _P]),
+    "lpa_ici_forward_noalloc": (_I, [_P, ctypes.POINTER(LpaImage), _I, ctypes.c_uint, _P]),
     "lpa_plan_create": (_I, [_P, ctypes.POINTER(LpaImage), _I, ctypes.c_uint, ctypes.POINTER(_P)]),
     "lpa_plan_run": (_I, [_P, _P]),
     "lpa_plan_destroy": (None, [_P]),
