@@ -184,4 +184,21 @@ inline Settings parse_command_line(int argc, char **argv, const std::string &ver
 
 }  // namespace lpa_cli
 
+// The reference's names for the same settings and parser (src/Config.hpp:7-14,
+// 22-24, 39), so a main() written against them builds unchanged.
+struct Config {
+    std::string image_file, kernel_file, output_file;
+    float sigma_noise;
+    unsigned int noise_seed;
+    float ici_gamma;
+};
+
+inline Config parse_args(int argc, char **argv, const std::string &version)
+{
+    const lpa_cli::Settings s = lpa_cli::parse_command_line(argc, argv, version);
+    return Config{s.image, s.kernels, s.output, s.sigma, s.seed, s.gamma};
+}
+
+inline std::string sibling_path(const std::string &path, const std::string &name) { return lpa_cli::beside(path, name); }
+
 #endif
