@@ -2,34 +2,44 @@
 
 Headline (one JSON line on stdout): plain form, one 4096x4096 fp32 synthetic
 image (sigma 20/255, gamma 2, 16x7 symmetric m=(1,0) kernel set), Mpixel/s
-over K timed forward passes with the input resident in HBM.  The same line
-carries
-  roofline      useful-tap FP32 FLOP/s of the forward kernel (HIP events on
-                its stream) against the MI355X vector FP32 peak
-  cpu_baseline  the reference's own v1_0 CPU code (oracle/_ref, 1 core) on a
-                bounded crop of the same input, or our restatement if the
-                reference build is absent
+over K timed forward passes with the input resident in HBM (K steps between
+two HIP events on the launch stream; each step also has its own event pair,
+whose median is reported beside the mean).  The same line carries
+  roofline      useful-tap FP32 FLOP/s of the forward kernel against the
+                MI355X vector FP32 peak
+  reference_window  the reference's timing window on the same image (from
+                after H2D to after D2H: the forward plus the copy of y into
+                pageable host memory), median of 5
+  float_ring_walk, weighted_walk  the other two fast walks on the same image
   gaussian_m20  the same image with the 16x7 m=(2,0) gaussian-window set
   batched_uneven  256 images with sides U{128..2048}, per-image sigma/gamma:
                 plain (one launch per image) vs one persistent launch vs its
                 hipGraph replay, each form at its own autotuned tile
-                configuration
+                configuration; the batched form's useful-tap rate
   fwd_bwd       64 x 512^2 forward + adjoint (dL/dx, dL/dkernel); the forward
                 saves the uint8 scale maps and the adjoint recomputes the
                 estimates from them; timed again with the forward keeping
                 each direction's selected estimate for the adjoint to read
                 ("kept_estimates"); the forward also one launch per image
   config5       32 images with sides U{1024..8192} (about 630 Mpx), fwd+bwd
-                in the same two forms, and each form's peak device memory in
-                use against the 288 GB of HBM
+                in the same two forms, peak device memory against 288 GB
+  cpu_baseline  the CPU leg: the reference's own v0_0 and v1_0 code
+                (oracle/_ref, one core each) on the 512^2 image and a 1024^2
+                image, the 4096^2 rate extrapolated from them and so
+                labelled, plus the oracle's v1_0-order walk on every host
+                core; and the accuracy of every GPU leg above against the
+                oracle on windows (scale agreement, |dy| where the scales
+                agree, dPSNR, error against the FP64 walk, adjoint error)
 Flags --skip-* drop the side legs.
 """
 import argparse
 import json
 import os
 import platform
+import statistics
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "lpa-ici-cuda_amd"))
@@ -38,10 +48,11 @@ import numpy as np  # noqa: E402
 import torch  # noqa: E402
 
 import lpa_ici  # noqa: E402
+from lpa_ici import workloads  # noqa: E402
 from lpa_ici.metrics import MI355X_FP32_PEAK_TFLOPS, useful_taps  # noqa: E402
 
 METRIC = "Mpixel/s at 4096^2 plain vs batched-uneven; fwd+bwd Mpixel/s; % FP32 VALU peak"
-SIGMA, GAMMA, SEED = 20.0 / 255.0, 2.0, 20250910
+SIGMA, GAMMA = workloads.SIGMA, workloads.GAMMA
 
 
 def log(*a):
@@ -49,28 +60,32 @@ def log(*a):
 
 
 def timed(fn, steps, warmup, stream):
-    """Median-free total: W untimed calls, then K calls between two events on
-    `stream`, synchronized on both sides.  Returns ms per call."""
+    """W untimed calls, then K calls between two events on `stream`
+    (synchronized on both sides), each call also between its own pair.
+    Returns {"ms": mean per call over the K, "median_ms": median of the
+    per-call times, "min_ms"}."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(steps):
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for i in range(steps):
         fn()
-    e1.record(stream)
+        ev[i + 1].record(stream)
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps
+    per = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
+    return {"ms": ev[0].elapsed_time(ev[-1]) / steps, "median_ms": statistics.median(per), "min_ms": min(per)}
 
 
 def rounded(d):
-    """A leg's record with its floats to three decimals, nested records too."""
-    return {k: rounded(v) if isinstance(v, dict) else round(v, 3) if isinstance(v, float) else v
-            for k, v in d.items()}
-
-
-def make_image(w, h, seed, sigma, noise_seed):
-    return lpa_ici.make_noisy(lpa_ici.synth_image(w, h, seed), sigma, noise_seed)
+    """A record with its floats to four significant decimals, nested records too."""
+    if isinstance(d, dict):
+        return {k: rounded(v) for k, v in d.items()}
+    if isinstance(d, list):
+        return [rounded(v) for v in d]
+    if isinstance(d, float):
+        return float(f"{d:.5g}")
+    return d
 
 
 # VALU lane-operations per second without packed math: 256 CUs x 2.4 GHz x
@@ -96,42 +111,86 @@ def executed_ops(ks, maps, walk):
     return 2 * useful_taps(maps, ks.ring_runs_per_kernel(), ks.S), "i32 add"
 
 
+def windows(H, W, n, count, seed):
+    """Corner, edge and random n x n windows of an H x W image."""
+    rng = np.random.default_rng(seed)
+    h, w = min(n, H), min(n, W)
+    out = [(0, 0), (H - h, W - w), (0, W - w), ((H - h) // 2, 0)]
+    out += [(int(rng.integers(0, H - h + 1)), int(rng.integers(0, W - w + 1))) for _ in range(max(0, count - 4))]
+    return [(r, c, h, w) for r, c in out[:count]]
+
+
+def window_data(x_host, y, maps, clean, wins, sigma=SIGMA, gamma=GAMMA):
+    """What the CPU leg needs to check a forward on windows (host copies)."""
+    y = y.cpu().numpy() if torch.is_tensor(y) else y
+    maps = maps.cpu().numpy() if torch.is_tensor(maps) else maps
+    parts = []
+    for r0, c0, h, w in wins:
+        parts.append({"win": (r0, c0, h, w), "y": y[r0:r0 + h, c0:c0 + w].copy(),
+                      "maps": maps[:, r0:r0 + h, c0:c0 + w].copy(),
+                      "clean": None if clean is None else clean[r0:r0 + h, c0:c0 + w].copy()})
+    return {"x": x_host, "sigma": sigma, "gamma": gamma, "parts": parts}
+
+
 def plain_leg(ks, x_host, steps, warmup, stream, label, flags=0):
     x = torch.from_numpy(x_host).cuda()
     y = torch.empty_like(x)
     maps = torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda")
     plan = lpa_ici.Plan(ks, [x], [y], [maps], SIGMA, GAMMA, flags=lpa_ici.FWD_PLAIN | flags)
     with torch.cuda.stream(stream):
-        ms = timed(lambda: plan.run(stream), steps, warmup, stream)
+        t = timed(lambda: plan.run(stream), steps, warmup, stream)
+    ms = t["ms"]
     taps = useful_taps(maps, ks.runs()["nnz"], ks.S)
     walk = walk_of(ks, flags)
     ex, op = executed_ops(ks, maps, walk)
     npx = x.numel()
     tflops = 2.0 * taps / (ms * 1e-3) / 1e12
     ex_rate = ex / (ms * 1e-3)
-    log(f"[{label}] {x.shape[1]}x{x.shape[0]}: {ms:.3f} ms/pass, {npx / ms / 1e3:.1f} Mpx/s, "
-        f"{taps / npx:.0f} useful taps/px ({tflops:.2f} TFLOP/s algorithmic), {walk}: {ex / npx:.0f} {op}/px "
-        f"executed ({ex_rate / MI355X_LANE_OPS:.3f} of the VALU lane-op rate)")
+    log(f"[{label}] {x.shape[1]}x{x.shape[0]}: {ms:.3f} ms/pass (median {t['median_ms']:.3f}), "
+        f"{npx / ms / 1e3:.1f} Mpx/s, {taps / npx:.0f} useful taps/px ({tflops:.2f} TFLOP/s algorithmic), "
+        f"{walk}: {ex / npx:.0f} {op}/px executed ({ex_rate / MI355X_LANE_OPS:.3f} of the VALU lane-op rate)")
     plan.close()
-    return {"ms": ms, "mpx_s": npx / ms / 1e3, "useful_taps": taps, "taps_per_px": taps / npx, "tflops": tflops,
-            "executed_op": op, "executed_ops_per_px": ex / npx, "executed_tops": ex_rate / 1e12,
-            "executed_frac_of_lane_ops": ex_rate / MI355X_LANE_OPS, "walk": walk, "y": y, "x": x}
+    return {"ms": ms, "median_ms": t["median_ms"], "mpx_s": npx / ms / 1e3, "useful_taps": taps,
+            "taps_per_px": taps / npx, "tflops": tflops, "executed_op": op, "executed_ops_per_px": ex / npx,
+            "executed_tops": ex_rate / 1e12, "executed_frac_of_lane_ops": ex_rate / MI355X_LANE_OPS, "walk": walk,
+            "y": y, "maps": maps, "x": x}
 
 
-def batched_leg(ks, steps, warmup, stream, n_images=256, seed=3):
-    rng = np.random.default_rng(seed)
-    sides = rng.integers(128, 2049, size=(n_images, 2))
-    sig = [float(rng.choice([10, 20, 30, 40])) / 255.0 for _ in range(n_images)]
-    gam = [float(rng.choice([1.5, 2.0, 2.5])) for _ in range(n_images)]
+def reference_window(ks, x_host, stream, reps=5):
+    """The reference's timing window (src/v4_1_lpa_ici_2D_sparse_shared.cu:
+    1216-1240): the forward on a resident image, then y copied into pageable
+    host memory; wall time, median of `reps` after one warm-up."""
+    x = torch.from_numpy(x_host).cuda()
+    y = torch.empty_like(x)
+    host = torch.empty(tuple(x.shape), dtype=torch.float32)  # pageable, as the reference's std::vector
+    plan = lpa_ici.Plan(ks, [x], [y], None, SIGMA, GAMMA, flags=lpa_ici.FWD_PLAIN)
+    times = []
+    with torch.cuda.stream(stream):
+        for i in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plan.run(stream)
+            host.copy_(y)
+            torch.cuda.synchronize()
+            if i:
+                times.append((time.perf_counter() - t0) * 1e3)
+    plan.close()
+    ms = statistics.median(times)
+    return {"ms": ms, "mpx_s": x.numel() / ms / 1e3, "runs": reps}
+
+
+def batched_leg(ks, steps, warmup, stream):
+    ws = workloads.uneven(256)
     t0 = time.time()
-    xs = [torch.from_numpy(make_image(int(w), int(h), 1000 + i, s, SEED + i)).cuda()
-          for i, ((w, h), s) in enumerate(zip(sides, sig))]
-    log(f"[batched] generated {n_images} images, {sum(x.numel() for x in xs) / 1e6:.1f} Mpx in {time.time() - t0:.1f} s")
-    npx = sum(x.numel() for x in xs)
+    xs_host = ws.noisy_all()
+    xs = [torch.from_numpy(x).cuda() for x in xs_host]
+    log(f"[batched] generated {len(ws)} images, {ws.pixels / 1e6:.1f} Mpx in {time.time() - t0:.1f} s")
+    sig, gam = ws.sigmas, ws.gammas
+    npx = ws.pixels
     ys_plain = [torch.empty_like(x) for x in xs]
     ys_batch = [torch.empty_like(x) for x in xs]
     ys_graph = [torch.empty_like(x) for x in xs]
-    out = {"images": n_images, "mpx": npx / 1e6}
+    out = {"images": len(ws), "mpx": npx / 1e6}
     # each form at the tile configuration the autotuner picks for it on this
     # workload (a plan keeps the configuration it was built with)
     from lpa_ici.autotune import autotune
@@ -141,30 +200,53 @@ def batched_leg(ks, steps, warmup, stream, n_images=256, seed=3):
     batch_cfg, _ = autotune(ks, xs, flags=0, sigmas=sig, gammas=gam, stream=stream)
     plans["batched"] = lpa_ici.Plan(ks, xs, ys_batch, None, sig, gam, flags=0)
     plans["graph"] = lpa_ici.Plan(ks, xs, ys_graph, None, sig, gam, flags=lpa_ici.FWD_GRAPH)
-    ks.set_tile_config(saved)
     out["tile_config"] = {"plain": plain_cfg, "batched": batch_cfg}
     log(f"[batched] tile configurations: plain {plain_cfg}, batched and graph {batch_cfg}")
     with torch.cuda.stream(stream):
         for name, plan in plans.items():
-            ms = timed(lambda: plan.run(stream), steps, warmup, stream)
-            out[name] = {"ms": ms, "mpx_s": npx / ms / 1e3}
-            log(f"[batched] {name}: {ms:.2f} ms, {npx / ms / 1e3:.1f} Mpx/s")
+            t = timed(lambda: plan.run(stream), steps, warmup, stream)
+            out[name] = {"ms": t["ms"], "median_ms": t["median_ms"], "mpx_s": npx / t["ms"] / 1e3}
+            log(f"[batched] {name}: {t['ms']:.2f} ms (median {t['median_ms']:.2f}), {npx / t['ms'] / 1e3:.1f} Mpx/s")
     torch.cuda.synchronize()
     out["identical"] = all(torch.equal(a, b) and torch.equal(a, c) for a, b, c in zip(ys_plain, ys_batch, ys_graph))
     out["batched_over_plain"] = out["batched"]["mpx_s"] / out["plain"]["mpx_s"]
     out["graph_over_plain"] = out["graph"]["mpx_s"] / out["plain"]["mpx_s"]
     for p in plans.values():
         p.close()
-    return out
+    # useful taps of the batch, from the scale maps of one more (untimed)
+    # batched run at the same configuration: the batched form's rate on them
+    maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda") for x in xs]
+    ys_m = [torch.empty_like(x) for x in xs]
+    pm = lpa_ici.Plan(ks, xs, ys_m, maps, sig, gam, flags=0)
+    with torch.cuda.stream(stream):
+        pm.run(stream)
+    torch.cuda.synchronize()
+    pm.close()
+    ks.set_tile_config(saved)
+    nnz = ks.runs()["nnz"]
+    taps = sum(useful_taps(m, nnz, ks.S) for m in maps)
+    for name in ("batched", "graph", "plain"):
+        tf = 2.0 * taps / (out[name]["ms"] * 1e-3) / 1e12
+        out[name]["tflops"] = tf
+        out[name]["frac"] = tf / MI355X_FP32_PEAK_TFLOPS
+    out["useful_taps_per_px"] = taps / npx
+    out["y_identical_with_maps"] = all(torch.equal(a, b) for a, b in zip(ys_m, ys_batch))
+    log(f"[batched] {taps / npx:.0f} useful taps/px: batched {out['batched']['tflops']:.1f} TFLOP/s "
+        f"({out['batched']['frac']:.3f} of peak)")
+    area = np.array([w * h for w, h in ws.sides])
+    pick = [int(i) for i in list(np.argsort(area)[:2]) + list(np.argsort(area)[-2:])]
+    check = [window_data(xs_host[i], ys_batch[i], maps[i], ws.clean(i), windows(*xs_host[i].shape, 96, 3, i),
+                         ws.sigmas[i], ws.gammas[i]) for i in pick]
+    return out, check
 
 
-def adjoint_variant(ks, xs, ys, maps, sig, gam, dys, dxs, dk, steps, warmup, stream, variant, on_step=None):
+def adjoint_variant(ks, xs, ys, maps, sig, gam, dys, dxs, dk, steps, warmup, stream, variant):
     """Times forward and forward + adjoint over the images, in one of two
     forms: "recompute" as the workload states it (the forward saves only the
     uint8 scale maps and the adjoint recomputes each direction's estimate from
     them), or "kept_estimates" (the forward also keeps each direction's
     selected estimate, 16 float maps per image, for the adjoint to read).
-    Returns (fwd_ms, fwd_bwd_ms)."""
+    Returns (forward timing, forward + adjoint timing)."""
     import ctypes
     n = len(xs)
     sig = sig if isinstance(sig, list) else [sig] * n
@@ -182,30 +264,49 @@ def adjoint_variant(ks, xs, ys, maps, sig, gam, dys, dxs, dk, steps, warmup, str
         fwd.run(stream)
         lpa_ici.check(lpa_ici.lib().lpa_ici_backward(ks.handle, arr, dy_p, dx_p, ctypes.c_void_p(dk.data_ptr()),
                                                      n, sp))
-        if on_step:
-            on_step()
 
     with torch.cuda.stream(stream):
-        ms_f = timed(lambda: fwd.run(stream), steps, warmup, stream)
-        ms_fb = timed(step, steps, warmup, stream)
-    if on_step:
-        torch.cuda.synchronize()
-        on_step()
+        t_f = timed(lambda: fwd.run(stream), steps, warmup, stream)
+        t_fb = timed(step, steps, warmup, stream)
+    torch.cuda.synchronize()
     fwd.close()
-    return ms_f, ms_fb
+    del ests
+    return t_f, t_fb
 
 
-def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
-    xs = [torch.from_numpy(make_image(side, side, 2000 + i, SIGMA, SEED + i)).cuda() for i in range(n)]
+def adjoint_record(ks, npx, sel_taps, t_f, t_fb):
+    ms_f, ms_fb = t_f["ms"], t_fb["ms"]
+    ms_b = ms_fb - ms_f
+    return {"fwd_ms": ms_f, "fwd_bwd_ms": ms_fb, "fwd_bwd_median_ms": t_fb["median_ms"],
+            "fwd_mpx_s": npx / ms_f / 1e3, "fwd_bwd_mpx_s": npx / ms_fb / 1e3, "bwd_ms": ms_b,
+            "bwd_tflops_algorithmic": 8.0 * sel_taps / (ms_b * 1e-3) / 1e12}
+
+
+def selected_taps(ks, maps):
+    nnz = torch.from_numpy(ks.runs()["nnz"].astype(np.int64).reshape(ks.D, ks.S)).cuda()
+    dsel = torch.arange(ks.D, device="cuda")[:, None, None]
+    return int(sum(nnz[dsel, m.long()].sum().item() for m in maps))
+
+
+def adjoint_check(x_host, g, maps, dx, sigma, wins):
+    """What the CPU leg needs to check dL/dx on windows (host copies)."""
+    return {"x": x_host, "g": g.cpu().numpy(), "maps": maps.cpu().numpy(), "dx": dx.cpu().numpy(), "sigma": sigma,
+            "wins": wins}
+
+
+def fwd_bwd_leg(ks, steps, warmup, stream):
+    ws = workloads.batch_512(64)
+    xs_host = ws.noisy_all()
+    xs = [torch.from_numpy(x).cuda() for x in xs_host]
     ys = [torch.empty_like(x) for x in xs]
-    maps = [torch.empty((ks.D, side, side), dtype=torch.uint8, device="cuda") for _ in xs]
+    maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda") for x in xs]
     g = torch.Generator(device="cuda").manual_seed(5)
     dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
     dxs = [torch.empty_like(x) for x in xs]
     dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
     times = {v: adjoint_variant(ks, xs, ys, maps, SIGMA, GAMMA, dys, dxs, dk, steps, warmup, stream, v)
              for v in ("recompute", "kept_estimates")}
-    npx = n * side * side
+    npx = ws.pixels
     # the same forward, one launch per image, at the plain form's own tile
     # configuration for these images
     from lpa_ici.autotune import autotune
@@ -214,32 +315,86 @@ def fwd_bwd_leg(ks, steps, warmup, stream, n=64, side=512):
     plain = lpa_ici.Plan(ks, xs, ys, maps, SIGMA, GAMMA, flags=lpa_ici.FWD_PLAIN)
     ks.set_tile_config(saved)
     with torch.cuda.stream(stream):
-        ms_plain = timed(lambda: plain.run(stream), steps, warmup, stream)
+        ms_plain = timed(lambda: plain.run(stream), steps, warmup, stream)["ms"]
     plain.close()
     log(f"[fwd+bwd] forward one launch per image: {ms_plain:.2f} ms (tile configuration {plain_cfg}), "
-        f"persistent {times['recompute'][0]:.2f} ms: {ms_plain / times['recompute'][0]:.2f}x")
+        f"persistent {times['recompute'][0]['ms']:.2f} ms: {ms_plain / times['recompute'][0]['ms']:.2f}x")
     # Algorithmic adjoint work: per selected kernel tap of every pixel and
     # direction, 2 FLOP to recompute yhat, 2 for the dL/dx gather and 4 for
     # dL/dkernel (the reference's taps, whatever the walks execute).
-    nnz = torch.from_numpy(ks.runs()["nnz"].astype(np.int64).reshape(ks.D, ks.S)).cuda()
-    dsel = torch.arange(ks.D, device="cuda")[:, None, None]
-    sel_taps = int(sum(nnz[dsel, m.long()].sum().item() for m in maps))
-    rec = {"images": n, "side": side, "selected_taps_per_px": sel_taps / npx,
+    sel = selected_taps(ks, maps)
+    rec = {"images": len(ws), "side": 512, "selected_taps_per_px": sel / npx,
            "fwd_plain_per_image_ms": ms_plain, "fwd_plain_tile_config": plain_cfg,
-           "fwd_batched_over_plain": ms_plain / times["recompute"][0]}
-    for variant, (ms_f, ms_fb) in times.items():
-        ms_b = ms_fb - ms_f
-        bwd_tflops = 8.0 * sel_taps / (ms_b * 1e-3) / 1e12
-        log(f"[fwd+bwd {variant}] {n}x{side}^2: fwd {ms_f:.2f} ms, fwd+bwd {ms_fb:.2f} ms, "
-            f"{npx / ms_fb / 1e3:.1f} Mpx/s; adjoint {ms_b:.2f} ms, {sel_taps / npx:.0f} selected taps/px, "
-            f"{bwd_tflops:.1f} TFLOP/s algorithmic")
-        v = {"fwd_ms": ms_f, "fwd_bwd_ms": ms_fb, "fwd_mpx_s": npx / ms_f / 1e3, "fwd_bwd_mpx_s": npx / ms_fb / 1e3,
-             "bwd_ms": ms_b, "bwd_tflops_algorithmic": bwd_tflops}
+           "fwd_batched_over_plain": ms_plain / times["recompute"][0]["ms"]}
+    for variant, (t_f, t_fb) in times.items():
+        v = adjoint_record(ks, npx, sel, t_f, t_fb)
+        log(f"[fwd+bwd {variant}] 64x512^2: fwd {v['fwd_ms']:.2f} ms, fwd+bwd {v['fwd_bwd_ms']:.2f} ms, "
+            f"{v['fwd_bwd_mpx_s']:.1f} Mpx/s; adjoint {v['bwd_ms']:.2f} ms, {sel / npx:.0f} selected taps/px, "
+            f"{v['bwd_tflops_algorithmic']:.1f} TFLOP/s algorithmic")
         if variant == "recompute":
             rec.update(v)
         else:
             rec[variant] = v
-    return rec
+    # the last timed step left dL/dx of the last call; recompute image 0's
+    # gradient alone for the check (dL/dx does not depend on dkernel)
+    dx0, _ = lpa_ici.backward(ks, [xs[0]], [ys[0]], [maps[0]], [dys[0]], SIGMA, GAMMA)
+    check = {"forward": window_data(xs_host[0], ys[0], maps[0], ws.clean(0), windows(512, 512, 128, 3, 1)),
+             "adjoint": adjoint_check(xs_host[0], dys[0], maps[0], dx0[0], SIGMA, windows(512, 512, 48, 2, 2))}
+    return rec, check
+
+
+def config5_leg(ks, steps, warmup, stream):
+    """32 images with sides U{1024..8192}, per-image sigma/gamma as the batched
+    leg: forward (scale maps kept as uint8) + adjoint that recomputes the
+    estimates from them, then the same with the estimates kept.  Peak device
+    memory per form: the tensors' high-water mark plus the library
+    workspaces' (an upper bound of the simultaneous peak)."""
+    ws = workloads.large_uneven(32)
+    torch.cuda.synchronize()
+    total = torch.cuda.mem_get_info()[1]
+    t0 = time.time()
+    xs_host = ws.noisy_all()
+    xs = [torch.from_numpy(x).cuda() for x in xs_host]
+    npx = ws.pixels
+    log(f"[config5] generated {len(ws)} images, {npx / 1e6:.1f} Mpx in {time.time() - t0:.1f} s")
+    sig, gam = ws.sigmas, ws.gammas
+    ys = [torch.empty_like(x) for x in xs]
+    maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda") for x in xs]
+    g = torch.Generator(device="cuda").manual_seed(7)
+    dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
+    dxs = [torch.empty_like(x) for x in xs]
+    dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
+    rec = {"images": len(ws), "mpx": round(npx / 1e6, 1), "hbm_total_gib": round(total / 2**30, 1),
+           "scale_maps_gib": round(npx * ks.D / 2**30, 2)}
+    sel = None
+    # the recompute form runs first, before any estimates exist, so each
+    # form's peak is its own
+    for variant in ("recompute", "kept_estimates"):
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        lpa_ici.workspace_peak(reset=True)
+        t_f, t_fb = adjoint_variant(ks, xs, ys, maps, sig, gam, dys, dxs, dk, steps, warmup, stream, variant)
+        tensors, work = torch.cuda.max_memory_allocated(), lpa_ici.workspace_peak()
+        if sel is None:
+            sel = selected_taps(ks, maps)
+        v = adjoint_record(ks, npx, sel, t_f, t_fb)
+        v["peak_hbm_gib"] = (tensors + work) / 2**30
+        v["peak_workspace_gib"] = work / 2**30
+        log(f"[config5 {variant}] {len(ws)} images {npx / 1e6:.1f} Mpx: fwd {v['fwd_ms']:.1f} ms "
+            f"({v['fwd_mpx_s']:.0f} Mpx/s), fwd+bwd {v['fwd_bwd_ms']:.1f} ms ({v['fwd_bwd_mpx_s']:.0f} Mpx/s); "
+            f"peak device memory <= {v['peak_hbm_gib']:.1f} GiB (library workspaces {work / 2**30:.2f}) of "
+            f"{total / 2**30:.0f} GiB")
+        if variant == "recompute":
+            rec.update(v)
+        else:
+            rec[variant] = v
+    i = int(np.argmax([w * h for w, h in ws.sides]))
+    H, W = xs_host[i].shape
+    dx1, _ = lpa_ici.backward(ks, [xs[i]], [ys[i]], [maps[i]], [dys[i]], sig[i], gam[i])
+    check = {"forward": window_data(xs_host[i], ys[i], maps[i], ws.clean(i), windows(H, W, 128, 3, 3), sig[i],
+                                    gam[i]),
+             "adjoint": adjoint_check(xs_host[i], dys[i], maps[i], dx1[0], sig[i], windows(H, W, 48, 2, 4))}
+    return rec, check
 
 
 def measured_traffic(walk):
@@ -255,81 +410,136 @@ def measured_traffic(walk):
     return None
 
 
-def config5_leg(ks, steps, warmup, stream, n=32, seed=5):
-    """32 images with sides U{1024..8192}, per-image sigma/gamma as the batched
-    leg: forward (scale maps kept as uint8) + adjoint that recomputes the
-    estimates from them, then the same with the estimates kept; peak device
-    memory in use reported."""
-    rng = np.random.default_rng(seed)
-    sides = rng.integers(1024, 8193, size=(n, 2))
-    sig = [float(rng.choice([10, 20, 30, 40])) / 255.0 for _ in range(n)]
-    gam = [float(rng.choice([1.5, 2.0, 2.5])) for _ in range(n)]
-    torch.cuda.synchronize()
-    free0, total = torch.cuda.mem_get_info()
-    torch.cuda.reset_peak_memory_stats()
-    t0 = time.time()
-    xs = [torch.from_numpy(make_image(int(w), int(h), 3000 + i, s, SEED + i)).cuda()
-          for i, ((w, h), s) in enumerate(zip(sides, sig))]
-    npx = sum(x.numel() for x in xs)
-    log(f"[config5] generated {n} images, {npx / 1e6:.1f} Mpx in {time.time() - t0:.1f} s")
-    ys = [torch.empty_like(x) for x in xs]
-    maps = [torch.empty((ks.D,) + tuple(x.shape), dtype=torch.uint8, device="cuda") for x in xs]
-    g = torch.Generator(device="cuda").manual_seed(7)
-    dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
-    dxs = [torch.empty_like(x) for x in xs]
-    dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
-    free_min = {"recompute": free0, "kept_estimates": free0}
-    current = ["recompute"]
+# ---------------------------------------------------------------------------
+# The CPU leg: the only part of this file that loads oracle/ (the reference's
+# CPU builds as the baseline, the oracle as the checker of the GPU outputs)
+# ---------------------------------------------------------------------------
 
-    def sample():
-        free_min[current[0]] = min(free_min[current[0]], torch.cuda.mem_get_info()[0])
-
-    # the recompute form runs first, before any estimates exist, so each
-    # variant's peak is its own
-    times = {}
-    for variant in ("recompute", "kept_estimates"):
-        current[0] = variant
-        times[variant] = adjoint_variant(ks, xs, ys, maps, sig, gam, dys, dxs, dk, steps, warmup, stream, variant,
-                                         sample)
-    rec = {"images": n, "mpx": round(npx / 1e6, 1), "hbm_total_gib": round(total / 2**30, 1),
-           "scale_maps_gib": round(npx * ks.D / 2**30, 2)}
-    for variant, (ms_f, ms_fb) in times.items():
-        peak = free0 - free_min[variant]
-        log(f"[config5 {variant}] {n} images {npx / 1e6:.1f} Mpx: fwd {ms_f:.1f} ms ({npx / ms_f / 1e3:.0f} Mpx/s), "
-            f"fwd+bwd {ms_fb:.1f} ms ({npx / ms_fb / 1e3:.0f} Mpx/s); peak device memory in use "
-            f"{peak / 2**30:.1f} GiB of {total / 2**30:.0f} GiB")
-        v = {"fwd_ms": ms_f, "fwd_mpx_s": npx / ms_f / 1e3, "fwd_bwd_ms": ms_fb, "fwd_bwd_mpx_s": npx / ms_fb / 1e3,
-             "peak_hbm_gib": round(peak / 2**30, 2)}
-        if variant == "recompute":
-            rec.update(v)
-        else:
-            rec[variant] = v
-    rec["torch_peak_allocated_gib"] = round(torch.cuda.max_memory_allocated() / 2**30, 2)
-    return rec
-
-
-def cpu_baseline(x_host, crop=640):
-    """The reference's v1_0 CPU code (fastest of its two CPU versions) on a
-    crop of the workload image, one core; our restatement if the reference
-    build is not present."""
+def cpu_leg(checks, ks_by_name, threads):
     sys.path.insert(0, REPO)
     import oracle
-    ks = lpa_ici.load_kernels()
+
+    def crop_of(H, W, r0, c0, h, w, m=31):
+        ra, ca = max(0, r0 - m), max(0, c0 - m)
+        rb, cb = min(H, r0 + h + m), min(W, c0 + w + m)
+        return (slice(ra, rb), slice(ca, cb)), (slice(r0 - ra, r0 - ra + h), slice(c0 - ca, c0 - ca + w))
+
+    def forward_accuracy(ks, data):
+        """The north-star bar on windows: per-direction scale agreement with
+        the reference-order FP32 walk, flips only at margins < 1e-6, |dy|
+        where every direction agrees, dPSNR over the windows, and the error
+        against the FP64 walk (its own selection, and frozen at the GPU's)."""
+        x, sigma, gamma = data["x"], data["sigma"], data["gamma"]
+        H, W = x.shape
+        var = ks.variances(sigma)
+        agree = np.zeros(ks.D)
+        total = flips = 0
+        max_dy = e64 = e64f = 0.0
+        margin_ok = True
+        ys, yos, cl = [], [], []
+        for part in data["parts"]:
+            r0, c0, h, w = part["win"]
+            crop, inner = crop_of(H, W, r0, c0, h, w)
+            xc = np.ascontiguousarray(x[crop])
+            yo, mo, marg = oracle.walk_f32(xc, ks, var, gamma, threads=threads)
+            sl = (slice(None),) + inner
+            yo, mo, marg = yo[inner], mo[sl], marg[sl]
+            ok = part["maps"] == mo
+            agree += ok.reshape(ks.D, -1).sum(axis=1)
+            total += h * w
+            flips += int((~ok).sum())
+            margin_ok &= bool(np.all(marg[~ok] < 1e-6))
+            all_ok = ok.all(axis=0)
+            max_dy = max(max_dy, float(np.abs(part["y"].astype(np.float64) - yo)[all_ok].max(initial=0.0)))
+            y64 = oracle.walk_f64(xc, ks, sigma, gamma, threads=threads)[0][inner]
+            frozen = np.zeros((ks.D,) + xc.shape, np.uint8)
+            frozen[sl] = part["maps"]
+            y64f = oracle.walk_f64(xc, ks, sigma, gamma, frozen=frozen, threads=threads)[0][inner]
+            e64 = max(e64, float(np.abs(part["y"] - y64).max()))
+            e64f = max(e64f, float(np.abs(part["y"] - y64f).max()))
+            ys.append(part["y"].ravel())
+            yos.append(yo.ravel())
+            if part["clean"] is not None:
+                cl.append(part["clean"].ravel())
+        out = {"pixels": total, "scale_agreement_min": float((agree / total).min()), "flips": flips,
+               "flips_at_margin_below_1e-6": margin_ok, "max_abs_dy_where_agree": max_dy,
+               "max_abs_err_fp64": e64, "max_abs_err_fp64_frozen_scales": e64f}
+        if cl:
+            c = np.concatenate(cl)
+            out["dpsnr_db"] = abs(lpa_ici.psnr(c, np.concatenate(ys)) - lpa_ici.psnr(c, np.concatenate(yos)))
+        out["meets_bar"] = bool(out["scale_agreement_min"] >= 0.999 and margin_ok and max_dy <= 1e-5 and
+                                out.get("dpsnr_db", 0.0) <= 0.005)
+        return out
+
+    def adjoint_accuracy(ks, data):
+        """dL/dx against the FP64 adjoint on windows (a window's dL/dx depends
+        on g and the maps within 31 px), relative to its largest value."""
+        x, g, maps, dx = data["x"], data["g"], data["maps"], data["dx"]
+        H, W = x.shape
+        err = 0.0
+        for r0, c0, h, w in data["wins"]:
+            crop, inner = crop_of(H, W, r0, c0, h, w)
+            dx64, _ = oracle.adjoint_f64(np.ascontiguousarray(x[crop]), np.ascontiguousarray(g[crop]), ks,
+                                         data["sigma"], np.ascontiguousarray(maps[(slice(None),) + crop]))
+            ref = dx64[inner]
+            err = max(err, float(np.abs(dx[r0:r0 + h, c0:c0 + w] - ref).max() / np.abs(ref).max()))
+        return {"dx_max_rel_err_fp64": err, "windows": len(data["wins"])}
+
+    t0 = time.time()
+    accuracy = {}
+    for name, (set_name, kind, data) in checks.items():
+        ks = ks_by_name[set_name]
+        if kind == "forward":
+            accuracy[name] = forward_accuracy(ks, data)
+        elif kind == "forward_list":
+            parts = [forward_accuracy(ks, d) for d in data]
+            accuracy[name] = {"images": len(parts), "pixels": sum(p["pixels"] for p in parts),
+                              "scale_agreement_min": min(p["scale_agreement_min"] for p in parts),
+                              "flips": sum(p["flips"] for p in parts),
+                              "max_abs_dy_where_agree": max(p["max_abs_dy_where_agree"] for p in parts),
+                              "max_abs_err_fp64_frozen_scales": max(p["max_abs_err_fp64_frozen_scales"] for p in parts),
+                              "dpsnr_db_max": max(p.get("dpsnr_db", 0.0) for p in parts),
+                              "meets_bar": all(p["meets_bar"] for p in parts)}
+        else:
+            accuracy[name] = adjoint_accuracy(ks, data)
+    log(f"[cpu] accuracy checks in {time.time() - t0:.1f} s: " +
+        ", ".join(f"{k}: {'bar met' if v.get('meets_bar', True) else 'BAR MISSED'}" for k, v in accuracy.items()))
+
+    # baseline: the reference's own v0_0 and v1_0 builds, one core each, run
+    # side by side on four host cores, on the 512^2 image and on a 1024^2
+    # image of the same generator
+    ks = ks_by_name["symmetric"]
     var = ks.variances(SIGMA)
-    h, w = x_host.shape
-    sample = np.ascontiguousarray(x_host[(h - crop) // 2:(h + crop) // 2, (w - crop) // 2:(w + crop) // 2])
-    kind = "reference" if oracle.ref_available("v1_0") else "port"
-    # the reference's code prints progress lines on stdout: keep them off the JSON stream
+    x512 = workloads.single(512).noisy(0)
+    x1024 = workloads.single(1024).noisy(0)
+    kind = "reference" if oracle.ref_available("v0_0") and oracle.ref_available("v1_0") else "port"
+
+    def run(version, x):
+        t = time.perf_counter()
+        if kind == "reference":
+            oracle.ref_denoise(x, ks, var, GAMMA, version=version)
+        elif version == "v1_0":
+            oracle.v1_0(x, ks, var, GAMMA)
+        else:
+            oracle.v0_0(x, ks, var, GAMMA)
+        sec = time.perf_counter() - t
+        return {"seconds": sec, "mpx_s": x.size / sec / 1e6}
+
+    jobs = {("v0_0", 512): x512, ("v1_0", 512): x512, ("v0_0", 1024): x1024, ("v1_0", 1024): x1024}
     sys.stdout.flush()
     saved = os.dup(1)
-    os.dup2(2, 1)
+    os.dup2(2, 1)  # the reference's code prints progress lines: keep them off the JSON stream
     try:
-        t = time.time()
-        if kind == "reference":
-            oracle.ref_denoise(sample, ks, var, GAMMA, version="v1_0")
-        else:
-            oracle.v1_0(sample, ks, var, GAMMA)
-        sec = time.time() - t
+        t0 = time.time()
+        with ThreadPoolExecutor(max_workers=len(jobs)) as pool:
+            futs = {k: pool.submit(run, k[0], x) for k, x in jobs.items()}
+            res = {k: f.result() for k, f in futs.items()}
+        wall = time.time() - t0
+        # our v1_0-order walk (same bits as v0_0) split by rows over every host core
+        t = time.perf_counter()
+        oracle.walk_f32(x1024, ks, var, GAMMA, threads=threads)
+        all_cores = {"seconds": time.perf_counter() - t}
+        all_cores["mpx_s"] = x1024.size / all_cores["seconds"] / 1e6
     finally:
         os.dup2(saved, 1)
         os.close(saved)
@@ -341,9 +551,21 @@ def cpu_baseline(x_host, crop=640):
                 break
     except OSError:
         pass
-    return {"value": sample.size / sec / 1e6, "unit": "Mpixel/s", "cores": 1, "kind": kind,
-            "sample": f"v1_0 CPU path on the central {crop}x{crop} crop of the 4096^2 noisy image, "
-                      f"{sec:.1f} s; host {cpu}, nproc {os.cpu_count()}"}
+    v10 = res[("v1_0", 512)]
+    baseline = {
+        "value": v10["mpx_s"], "unit": "Mpixel/s", "cores": 1, "kind": kind,
+        "sample": f"the reference's v1_0 CPU path (its faster one) on the 512^2 synthetic image, sigma 20/255, "
+                  f"gamma 2, symmetric set: {v10['seconds']:.1f} s; v0_0 and v1_0 at 512^2 and 1024^2 ran side by "
+                  f"side, one core each ({wall:.0f} s wall); host {cpu}, nproc {os.cpu_count()}",
+        "runs": {f"{v}_{n}": {"seconds": r["seconds"], "mpx_s": r["mpx_s"]} for (v, n), r in res.items()},
+        "extrapolated_4096_seconds": {v: 4096 * 4096 / (res[(v, 1024)]["mpx_s"] * 1e6) for v in ("v0_0", "v1_0")},
+        "extrapolation": "4096^2 seconds from the 1024^2 rate (per-pixel work does not grow with the image); "
+                         "not measured",
+        "oracle_walk_all_cores_1024": dict(all_cores, cores=threads, kind="port"),
+    }
+    log(f"[cpu] baseline: " + ", ".join(f"{v} {n}^2 {r['seconds']:.1f} s ({r['mpx_s']:.4f} Mpx/s)"
+                                         for (v, n), r in res.items()))
+    return baseline, accuracy
 
 
 def main():
@@ -352,12 +574,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true", help="no CPU baseline and no accuracy checks")
+    ap.add_argument("--skip-walks", action="store_true")
     ap.add_argument("--skip-gaussian", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-fwdbwd", action="store_true")
     ap.add_argument("--skip-config5", action="store_true")
     ap.add_argument("--tile-config", type=int, default=-1)
+    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (the reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -377,7 +601,9 @@ def main():
     stream = torch.cuda.Stream()
     ks = lpa_ici.load_kernels()
     t0 = time.time()
-    x_host = make_image(args.size, args.size, 42, SIGMA, SEED)
+    head = workloads.single(args.size)
+    clean = head.clean(0)
+    x_host = lpa_ici.make_noisy(clean, head.sigmas[0], head.noise_seeds[0])
     log(f"[setup] {args.size}^2 image in {time.time() - t0:.1f} s; device {torch.cuda.get_device_name(0)}")
     tune = {}
     if args.tile_config >= 0:
@@ -387,9 +613,11 @@ def main():
         best, times = autotune(ks, [torch.from_numpy(x_host).cuda()], flags=lpa_ici.FWD_PLAIN, stream=stream,
                                cache=None)
         tune = {str(k): round(v, 4) for k, v in times.items()}
-        log(f"[autotune] plain 4096^2: best config {best}; ms per config {tune}")
+        log(f"[autotune] plain {args.size}^2: best config {best}; ms per config {tune}")
 
     main_leg = plain_leg(ks, x_host, args.steps, args.warmup, stream, "plain symmetric")
+    wins = windows(args.size, args.size, 192, 8, 11)
+    checks = {"headline": ("symmetric", "forward", window_data(x_host, main_leg["y"], main_leg["maps"], clean, wins))}
     npx = args.size * args.size
     rec = {
         "metric": METRIC,
@@ -399,6 +627,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(main_leg["ms"], 4),
+        "median_ms_per_step": round(main_leg["median_ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -420,6 +649,11 @@ def main():
                      "executed_frac_of_lane_ops": round(main_leg["executed_frac_of_lane_ops"], 4)},
     }
     headline_config = ks.tile_config()
+    del main_leg
+    rw = reference_window(ks, x_host, stream)
+    rec["reference_window"] = rounded(dict(rw, what="forward + copy of y into pageable host memory, wall clock, "
+                                                    "median of 5 (the reference's timer window)"))
+    log(f"[reference window] {rw['ms']:.2f} ms, {rw['mpx_s']:.0f} Mpx/s")
 
     def tune_for(kset, flags, label):
         """Autotune the tile configuration for this walk (unless one was given)."""
@@ -427,40 +661,56 @@ def main():
             kset.set_tile_config(args.tile_config)
             return
         from lpa_ici.autotune import autotune
-        best, times = autotune(kset, [torch.from_numpy(x_host).cuda()], flags=lpa_ici.FWD_PLAIN | flags,
-                               stream=stream, cache=None)
+        best, _ = autotune(kset, [torch.from_numpy(x_host).cuda()], flags=lpa_ici.FWD_PLAIN | flags,
+                           stream=stream, cache=None)
         log(f"[autotune] {label}: best config {best}")
 
-    if main_leg["walk"] == "fixed-point ring sums":
+    if not args.skip_walks and ks.info()["ring_form"] == 1:
         for key, flags in (("float_ring_walk", lpa_ici.FWD_RING_FLOAT), ("weighted_walk", lpa_ici.FWD_WEIGHTED)):
             tune_for(ks, flags, key)
             w = plain_leg(ks, x_host, args.steps, args.warmup, stream, f"plain symmetric {key}", flags=flags)
-            rec[key] = {"mpx_s": round(w["mpx_s"], 2), "ms": round(w["ms"], 4), "tflops": round(w["tflops"], 3),
-                        "frac": round(w["tflops"] / MI355X_FP32_PEAK_TFLOPS, 4),
-                        "executed_op": w["executed_op"], "executed_ops_per_px": round(w["executed_ops_per_px"], 1),
-                        "executed_frac_of_lane_ops": round(w["executed_frac_of_lane_ops"], 4),
-                        "tile_config": ks.tile_config()}
+            rec[key] = rounded({"mpx_s": w["mpx_s"], "ms": w["ms"], "median_ms": w["median_ms"],
+                                "tflops": w["tflops"], "frac": w["tflops"] / MI355X_FP32_PEAK_TFLOPS,
+                                "executed_op": w["executed_op"], "executed_ops_per_px": w["executed_ops_per_px"],
+                                "executed_frac_of_lane_ops": w["executed_frac_of_lane_ops"],
+                                "tile_config": ks.tile_config()})
+            checks[key] = ("symmetric", "forward", window_data(x_host, w["y"], w["maps"], clean, wins[:4]))
+            del w
         ks.set_tile_config(headline_config)
-    if not args.skip_cpu:
-        rec["cpu_baseline"] = cpu_baseline(x_host)
-        log(f"[cpu] {rec['cpu_baseline']}")
+    ksg = None
     if not args.skip_gaussian:
         ksg = lpa_ici.load_kernels(lpa_ici.GAUSSIAN_M20_KERNELS)
         tune_for(ksg, 0, "m20 gaussian")
         g = plain_leg(ksg, x_host, args.steps, args.warmup, stream, "plain m20 gaussian")
-        rec["gaussian_m20"] = {"mpx_s": round(g["mpx_s"], 2), "ms": round(g["ms"], 4),
-                               "tflops": round(g["tflops"], 3), "taps_per_px": round(g["taps_per_px"], 1),
-                               "tile_config": ksg.tile_config()}
-    del main_leg
+        rec["gaussian_m20"] = rounded({"mpx_s": g["mpx_s"], "ms": g["ms"], "median_ms": g["median_ms"],
+                                       "tflops": g["tflops"], "frac": g["tflops"] / MI355X_FP32_PEAK_TFLOPS,
+                                       "taps_per_px": g["taps_per_px"], "executed_op": g["executed_op"],
+                                       "executed_frac_of_lane_ops": g["executed_frac_of_lane_ops"],
+                                       "tile_config": ksg.tile_config()})
+        checks["gaussian_m20"] = ("gaussian", "forward", window_data(x_host, g["y"], g["maps"], clean, wins))
+        del g
+    torch.cuda.empty_cache()
     if not args.skip_batched:
-        b = batched_leg(ks, max(3, args.steps // 4), 1, stream)
+        b, chk = batched_leg(ks, max(5, args.steps // 4), 1, stream)
         rec["batched_uneven"] = rounded(b)
+        checks["batched_uneven"] = ("symmetric", "forward_list", chk)
+        torch.cuda.empty_cache()
     if not args.skip_fwdbwd:
-        fb = fwd_bwd_leg(ks, max(3, args.steps // 4), 1, stream)
+        fb, chk = fwd_bwd_leg(ks, max(5, args.steps // 4), 1, stream)
         rec["fwd_bwd"] = rounded(fb)
+        checks["fwd_bwd_forward"] = ("symmetric", "forward", chk["forward"])
+        checks["fwd_bwd_adjoint"] = ("symmetric", "adjoint", chk["adjoint"])
+        torch.cuda.empty_cache()
     if not args.skip_config5:
-        c5 = config5_leg(ks, 2, 1, stream)
+        c5, chk = config5_leg(ks, 5, 1, stream)
         rec["config5"] = rounded(c5)
+        checks["config5_forward"] = ("symmetric", "forward", chk["forward"])
+        checks["config5_adjoint"] = ("symmetric", "adjoint", chk["adjoint"])
+        torch.cuda.empty_cache()
+    if not args.skip_cpu:
+        baseline, accuracy = cpu_leg(checks, {"symmetric": ks, "gaussian": ksg}, args.cpu_threads)
+        rec["cpu_baseline"] = rounded(baseline)
+        rec["accuracy"] = rounded(accuracy)
     rec["host"] = platform.node()
     print(json.dumps(rec), flush=True)
 
