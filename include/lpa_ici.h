@@ -191,8 +191,12 @@ int lpa_kset_get_tile_config(lpa_kset_t ks);
  * est[i] (device, [D][height][width] floats, dense) receives, per direction
  * d and pixel p, the estimate of d at scale scale_map[d][p] (the map is
  * required), computed with the forward's walk for the same flags
- * (LPA_FWD_EXACT, LPA_FWD_WEIGHTED, LPA_FWD_RING_FLOAT), so it is the value
- * that forward aggregated, bit for bit.  These are the reference's per
+ * (LPA_FWD_EXACT, LPA_FWD_WEIGHTED, LPA_FWD_RING_FLOAT) and, for the
+ * fixed-point ring walk, on the kernel set's tile configuration (that walk
+ * chooses fixed point or float ring sums per tile window), so it is the value
+ * a forward with the same flags and configuration aggregated, bit for bit.
+ * A forward can also keep these values as it walks (lpa_image.estimates).
+ * These are the reference's per
  * direction debugging outputs (the commented-out writes of img_denoised_d and
  * scale_map_d, src/v0_0_lpa_ici_2D_naive.cpp:848, 901, 906-914).  Errors:
  * LPA_EINVAL (missing scale map or est, LPA_FWD_GRAPH), LPA_ETOOBIG, LPA_ENOMEM,

@@ -673,7 +673,8 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
         if (!est[i]) return fail(LPA_EINVAL, "image " + std::to_string(i) + ": null est");
     }
     hipStream_t s = (hipStream_t)stream;
-    constexpr int SH = lpa::kEstimateShape;
+    const lpa::WalkMode mode = walk_mode(ks, flags);
+    const int SH = mode == lpa::kWalkRing ? shape_of(ks, flags) : lpa::kEstimateShape;
     std::vector<uint2> tiles;
     for (int i = 0; i < n; ++i) {
         const int tx = tiles_x_of(SH, imgs[i].width), ty = tiles_y_of(SH, imgs[i].height);
@@ -709,7 +710,7 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
     hipError_t e = hipMemcpyAsync(mem, host.data(), bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
     if (e == hipSuccess)
-        e = lpa::launch_estimates(walk_mode(ks, flags), ks->dev(), ks->has_rings ? ks->rings() : lpa::DevRings{},
+        e = lpa::launch_estimates(mode, SH, ks->dev(), ks->has_rings ? ks->rings() : lpa::DevRings{},
                                   (const lpa::EstDesc *)(base + off_desc), (const uint2 *)(base + off_tiles),
                                   (int)tiles.size(), (int *)base, ks->radius, grid, s);
     hipError_t ef = hipFreeAsync(mem, s);

@@ -138,16 +138,19 @@ hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, c
 int blocks_per_cu(int shape, int radius, int num_kernels);
 
 // Per-direction estimates at frozen scales (lpa_fwd.hip): est = [D][H][W]
-// floats per image, dense like the scale map.  Tile configuration
-// kEstimateShape, a persistent grid over the tile queue.
+// floats per image, dense like the scale map; a persistent grid over the tile
+// queue.  The fixed-point ring walk runs on the forward's tile configuration
+// (`shape`: its choice of fixed point or float per tile window must be the
+// forward's); every other walk gives the same bits on any configuration and
+// runs on kEstimateShape.
 struct EstDesc {
     ImgDesc img;
     float *est;
 };
 constexpr int kEstimateShape = 5;
-hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &rings, const EstDesc *d_imgs,
-                            const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
-                            hipStream_t stream);
+hipError_t launch_estimates(WalkMode mode, int shape, const DevKset &ks, const DevRings &rings,
+                            const EstDesc *d_imgs, const uint2 *d_tiles, int num_tiles, int *d_counter, int radius,
+                            int grid, hipStream_t stream);
 
 // Adjoint (lpa_bwd.hip).  The first pass walks 64 x 64 tiles (d_tiles), the
 // dx pass tiles 64 wide and kDxTileRowsRing (ring form) or

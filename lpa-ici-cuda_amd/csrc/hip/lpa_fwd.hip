@@ -136,6 +136,20 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
         if (inside[p]) yrow[p] = num[p] / den[p];
 }
 
+// The lane's cell of a tile: column tx (in units of P pixels) and row ty.
+// SPLIT: a wave covers 64 / LX rows; lanes 0-31 take the left LX / 2 lane
+// columns of them, lanes 32-63 the right LX / 2.
+template <int TW, int TH, int P, int NT, int SPLIT>
+__device__ __forceinline__ void lane_cell(int tid, int &tx, int &ty)
+{
+    constexpr int LX = TW / P;
+    static_assert(LX * TH == NT, "tile shape must use every lane of the work-group");
+    static_assert(LX >= 2 && (LX & (LX - 1)) == 0 && LX <= 64, "lanes per row: a power of two up to 64");
+    constexpr int HW = LX / 2, RW = 64 / LX;
+    tx = SPLIT ? ((tid & 32) ? HW : 0) + (tid & (HW - 1)) : tid % LX;
+    ty = SPLIT ? (tid >> 6) * RW + (tid & 31) / HW : tid / LX;
+}
+
 // One output tile at (x0, y0) of an image.  `tile` is the work-group's LDS
 // buffer of lds_floats(TW + 1, TH, R, PM) words: the window (tile plus a halo
 // of R) with one leading column, which the fixed-point form's prefix table
@@ -149,16 +163,10 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
                                              float *__restrict__ y, uint8_t *__restrict__ map,
                                              float *__restrict__ est, int W, int H, int pitch, int x0, int y0, int R)
 {
-    constexpr int LX = TW / P;
-    static_assert(LX * TH == NT, "tile shape must use every lane of the work-group");
-    static_assert(LX >= 2 && (LX & (LX - 1)) == 0 && LX <= 64, "lanes per row: a power of two up to 64");
     const int lp = lds_pitch(TW + 1, R, PM);
     const int tid = threadIdx.x;
-    // SPLIT: a wave covers 64 / LX rows; lanes 0-31 take the left LX / 2 lane
-    // columns of them, lanes 32-63 the right LX / 2.
-    constexpr int HW = LX / 2, RW = 64 / LX;
-    const int tx = SPLIT ? ((tid & 32) ? HW : 0) + (tid & (HW - 1)) : tid % LX;
-    const int ty = SPLIT ? (tid >> 6) * RW + (tid & 31) / HW : tid / LX;
+    int tx, ty;
+    lane_cell<TW, TH, P, NT, SPLIT>(tid, tx, ty);
     const int px0 = x0 + tx * P, py = y0 + ty;
     // LDS offset of window column R + tx*P of row ty + R, not counting the
     // leading column.  Past the leading column (the float tile) it is the
@@ -262,19 +270,22 @@ __device__ __forceinline__ void estimate_lane(const T *__restrict__ src, int lp,
 }
 
 // Persistent estimates kernel over a tile queue spanning a batch, on tile
-// configuration SH's shape.
+// configuration SH's shape.  For the fixed-point ring walk the shape is the
+// forward's own: the choice between fixed point and float ring sums is made
+// per tile window, so the same windows give the same choice.
 template <int SH, int MODE>
-__global__ __launch_bounds__(kThreads) void estimate_kernel(DevKset ks, DevRings rings,
-                                                            const EstDesc *__restrict__ imgs,
-                                                            const uint2 *__restrict__ tiles, int num_tiles,
-                                                            int *__restrict__ counter, int R)
+__global__ __launch_bounds__(kShapes[SH].nt) void estimate_kernel(DevKset ks, DevRings rings,
+                                                                  const EstDesc *__restrict__ imgs,
+                                                                  const uint2 *__restrict__ tiles, int num_tiles,
+                                                                  int *__restrict__ counter, int R)
 {
     extern __shared__ float tile[];
     __shared__ int next;
     constexpr TileShape sh = kShapes[SH];
-    constexpr int LX = sh.tw / sh.px;
     const int lp = lds_pitch(sh.tw + 1, R, sh.pm);
-    const int tid = threadIdx.x, tx = tid % LX, ty = tid / LX;
+    const int tid = threadIdx.x;
+    int tx, ty;
+    lane_cell<sh.tw, sh.th, sh.px, sh.nt, sh.split>(tid, tx, ty);
     const int lane_off = (ty + R) * lp + tx * sh.px + R;
     for (;;) {
         if (tid == 0) next = atomicAdd(counter, 1);
@@ -288,14 +299,14 @@ __global__ __launch_bounds__(kThreads) void estimate_kernel(DevKset ks, DevRings
         const int px0 = x0 + tx * sh.px, py = y0 + ty;
         bool fixed = false;
         if constexpr (MODE == kWalkRing)
-            fixed = stage_prefix<sh.tw, sh.th>((uint32_t *)tile, lp, img.x, img.W, img.H, img.pitch, x0, y0, R,
-                                               rings.qscale, rings.qrange);
+            fixed = stage_prefix<sh.tw, sh.th, sh.nt>((uint32_t *)tile, lp, img.x, img.W, img.H, img.pitch, x0, y0,
+                                                      R, rings.qscale, rings.qrange);
         if (fixed) {
             estimate_lane<sh.px, kWalkRing, sh.body>((const uint32_t *)tile + lane_off, lp, ks, rings, img.map,
                                                       ed.est, img.W, img.H, px0, py);
         } else {
             __syncthreads();  // every lane has voted before the window is restaged
-            stage_tile<sh.tw, sh.th>(tile + 1, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
+            stage_tile<sh.tw, sh.th, sh.nt>(tile + 1, lp, img.x, img.W, img.H, img.pitch, x0, y0, R);
             __syncthreads();
             constexpr int M = MODE == kWalkRing ? kWalkRingFloat : MODE;
             estimate_lane<sh.px, M, sh.body>(tile + 1 + lane_off, lp, ks, rings, img.map, ed.est, img.W, img.H, px0,
@@ -466,25 +477,44 @@ hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, c
     }
 }
 
-hipError_t launch_estimates(WalkMode mode, const DevKset &ks, const DevRings &rings, const EstDesc *d_imgs,
-                            const uint2 *d_tiles, int num_tiles, int *d_counter, int radius, int grid,
-                            hipStream_t stream)
+template <int SH, int MODE>
+hipError_t estimate_launch(const DevKset &ks, const DevRings &rings, const EstDesc *d_imgs, const uint2 *d_tiles,
+                           int num_tiles, int *d_counter, int radius, int grid, hipStream_t stream)
 {
-    constexpr int SH = kEstimateShape;
     constexpr TileShape sh = kShapes[SH];
     const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw + 1, sh.th, radius, sh.pm);
+    const hipError_t e = allow_lds(estimate_kernel<SH, MODE>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((estimate_kernel<SH, MODE>), dim3(grid), dim3(sh.nt), lds, stream, ks, rings, d_imgs, d_tiles,
+                       num_tiles, d_counter, radius);
+    return hipGetLastError();
+}
+
+hipError_t launch_estimates(WalkMode mode, int shape, const DevKset &ks, const DevRings &rings,
+                            const EstDesc *d_imgs, const uint2 *d_tiles, int num_tiles, int *d_counter, int radius,
+                            int grid, hipStream_t stream)
+{
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, d_counter);
-    switch (mode) {
-#define LPA_E(M)                                                                                                  \
-    case M:                                                                                                        \
-        hipLaunchKernelGGL((estimate_kernel<SH, M>), dim3(grid), dim3(kThreads), lds, stream, ks, rings, d_imgs,       \
-                           d_tiles, num_tiles, d_counter, radius);                                                 \
-        break;
-        LPA_E(kWalkFma) LPA_E(kWalkExact) LPA_E(kWalkRing) LPA_E(kWalkRingFloat)
+    if (mode == kWalkRing) {
+        switch (shape) {
+#define LPA_E(SH)                                                                                                  \
+    case SH: return estimate_launch<SH, kWalkRing>(ks, rings, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+            LPA_E(0) LPA_E(1) LPA_E(2) LPA_E(3) LPA_E(4) LPA_E(5) LPA_E(6) LPA_E(7) LPA_E(8) LPA_E(9)
 #undef LPA_E
+        default: return hipErrorInvalidValue;
+        }
+    }
+    if (shape != kEstimateShape) return hipErrorInvalidValue;
+    constexpr int SH = kEstimateShape;
+    switch (mode) {
+    case kWalkFma: return estimate_launch<SH, kWalkFma>(ks, rings, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case kWalkExact:
+        return estimate_launch<SH, kWalkExact>(ks, rings, d_imgs, d_tiles, num_tiles, d_counter, radius, grid, stream);
+    case kWalkRingFloat:
+        return estimate_launch<SH, kWalkRingFloat>(ks, rings, d_imgs, d_tiles, num_tiles, d_counter, radius, grid,
+                                                   stream);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 int blocks_per_cu(int shape, int radius, int num_kernels)

@@ -87,7 +87,7 @@ void anisotropic_lpa_ici_gpu_fused(float *host_img_noisy, float *host_img_denois
 // <dir>gray_image_denoised_theta_<d>.hdr (the estimate at the selected scale)
 // and <dir>scale_map_theta_<d>.hdr (s / (S - 1), as the reference's
 // scale_map_d, :848 and :901), from one forward that keeps the uint8 scale
-// map and lpa_ici_estimates.
+// map and each direction's selected estimate (the values it aggregated).
 void write_direction_outputs(const float *host_img_noisy, int width, int height, int num_dirs, int num_scales,
                              const float *host_kernel_weights, const int *host_offsets, const int *host_rows,
                              const int *host_cols, const float *host_kernel_variances, float ici_gamma,
@@ -112,9 +112,8 @@ void write_direction_outputs(const float *host_img_noisy, int width, int height,
     img.height = height;
     img.gamma = ici_gamma;
     img.variances = host_kernel_variances;
+    img.estimates = d_est;  // the forward keeps the estimates it aggregates
     check(lpa_ici_forward(ks, &img, 1, LPA_FWD_PLAIN, nullptr), "forward");
-    float *const est[1] = {d_est};
-    check(lpa_ici_estimates(ks, &img, est, 1, 0, nullptr), "estimates");
     std::vector<float> h_est(npx * num_dirs), plane(npx);
     std::vector<uint8_t> h_map(npx * num_dirs);
     check(hipMemcpy(h_est.data(), d_est, sizeof(float) * h_est.size(), hipMemcpyDeviceToHost), "hipMemcpy");

@@ -238,19 +238,48 @@ def test_adjoint_batched_uneven(ks):
     assert torch.allclose(dk, dk_sum, rtol=1e-4, atol=1e-4 * dk.abs().max().item())
 
 
-def test_autograd_harness(ks):
+@pytest.mark.parametrize("keep", [False, True])
+def test_autograd_harness(ks, keep):
+    """The autograd function equals the C ABI calls: y bit for bit, dL/dx
+    bit for bit, dL/dtaps up to float reassociation; with the uint8 maps
+    alone (the default) and with the kept estimates."""
     x = noisy_synth(40, 36, seed=9)
     xd = torch.from_numpy(x).cuda().requires_grad_(True)
     taps = template_taps(ks).requires_grad_(True)
-    y = lpa_ici_fn(xd, taps, ks, SIGMA, GAMMA)
+    y = lpa_ici_fn(xd, taps, ks, SIGMA, GAMMA, keep_estimates=keep)
     y_ref = lpa_ici.denoise(ks, torch.from_numpy(x).cuda(), SIGMA, GAMMA)
     assert torch.equal(y.detach(), y_ref)
     g = torch.randn_like(y)
     (y * g).sum().backward()
     _, m = lpa_ici.denoise(ks, xd.detach(), SIGMA, GAMMA, scale_maps=True)
     dxs, dk = lpa_ici.backward(ks, [xd.detach()], [y_ref], [m], [g], SIGMA, GAMMA)
-    assert torch.allclose(xd.grad, dxs[0], rtol=0, atol=0)
+    assert torch.equal(xd.grad, dxs[0])
     assert torch.allclose(taps.grad, dk, rtol=1e-5, atol=1e-6 * dk.abs().max().item())
+
+
+def test_autograd_pitched_input_and_set_reuse(ks):
+    """A column slice of a wider buffer goes through the autograd function
+    (rows pitched, unit stride along a row); the kernel set is built once
+    while the taps keep their values, and rebuilt when they change."""
+    from lpa_ici.autograd import kernel_set_for
+    x = noisy_synth(70, 50, seed=19)
+    buf = torch.full((50, 96), float("nan"), device="cuda")
+    buf[:, :70] = torch.from_numpy(x).cuda()
+    xv = buf[:, :70].requires_grad_(True)
+    taps = template_taps(ks).requires_grad_(True)
+    y = lpa_ici_fn(xv, taps, ks, SIGMA, GAMMA)
+    g = torch.randn_like(y)
+    (y * g).sum().backward()
+    xd = torch.from_numpy(x).cuda()
+    y0, m0 = lpa_ici.denoise(ks, xd, SIGMA, GAMMA, scale_maps=True)
+    dx0, _ = lpa_ici.backward(ks, [xd], [y0], [m0], [g.contiguous()], SIGMA, GAMMA)
+    assert torch.equal(y.detach().contiguous(), y0)
+    assert torch.equal(xv.grad.contiguous(), dx0[0])
+    first = kernel_set_for(ks, taps)
+    assert kernel_set_for(ks, taps.detach().clone()) is first
+    changed = taps.detach().clone()
+    changed[0] += 1e-3
+    assert kernel_set_for(ks, changed) is not first
 
 
 def test_backward_needs_scale_map(ks):

@@ -351,6 +351,26 @@ def test_estimates_fixed_point_bitexact(ks):
     assert np.array_equal(est.view(np.uint32), eo.view(np.uint32))
 
 
+@pytest.mark.parametrize("shape", [3, 5, 8, 9])
+def test_estimates_follow_the_forward_tile_config(ks, shape):
+    """With a block of values outside the fixed-point range, windows that
+    reach it take float ring sums and the others fixed point: the estimates
+    are taken on the forward's tile configuration, so they equal the values
+    the forward kept, bit for bit, whatever the configuration."""
+    _, rng = fixed_point(ks)
+    x = noisy_synth(300, 200, seed=21)
+    x[60:70, 100:110] = 3.0 * rng
+    xd = dev(x)
+    ks.set_tile_config(shape)
+    try:
+        _, m, e = lpa_ici.denoise(ks, xd, SIGMA, GAMMA, scale_maps=True, keep_estimates=True)
+        e2 = lpa_ici.estimates(ks, xd, m)
+        torch.cuda.synchronize()
+    finally:
+        ks.set_tile_config(-1)
+    assert torch.equal(e.view(torch.int32), e2.view(torch.int32))
+
+
 def test_estimates_need_a_scale_map(ks):
     xd = dev(noisy_synth(16, 16, seed=1))
     import ctypes
