@@ -258,6 +258,27 @@ lpa_status lpa_load_kernels(const char* path, int* num_dirs, int* num_scales, in
 lpa_status lpa_wedge_kernels(int num_dirs, int num_scales, const int* h, int* n_weights, float* weights,
                              int* offsets, int* rows, int* cols);
 
+/* Directional LPA kernel sets of polynomial order `order` (no counterpart in
+ * the reference's code: its sets come precomputed from a modified LASIP
+ * demo_CreateLPAKernels.m, readme.md:87-92, 239).  Kernel (d, s) is the
+ * zero-order LPA estimator: a degree-min(order, h-1) polynomial P fitted by
+ * weighted least squares on the samples u = 0..h[s]-1 along direction
+ * phi_d = 2 pi d / D, then k = w(u) w(v) P(u) on the sector wedge of
+ * lpa_wedge_kernels ((u, v) the point's coordinates along and across the
+ * direction), divided by its sum.  window: LPA_WINDOW_UNIFORM (w = 1),
+ * LPA_WINDOW_GAUSSIAN (w(t) = exp(-t^2 / (2 (0.3 h)^2))), or
+ * LPA_WINDOW_SQUARE (D = 4 only: the h x h quadrant to the right of the
+ * direction, fitted on all its points).  Weights with six decimals, as the
+ * files store them.  order 0 with the uniform window is lpa_wedge_kernels.
+ * The square and symmetric sets, and the axis directions of the m = 1 and
+ * m = 2 sets, come out equal to the shipped files; oblique directions of
+ * m >= 1 sets differ from them (LASIP's discretization there is not known).
+ * Needs 0 <= order <= 4, 1 <= h <= 32, D >= 4.  Same two-call protocol and
+ * layout as lpa_load_kernels. */
+enum { LPA_WINDOW_UNIFORM = 0, LPA_WINDOW_GAUSSIAN = 1, LPA_WINDOW_SQUARE = 2 };
+lpa_status lpa_make_kernels(int num_dirs, int num_scales, const int* h, int order, int window, int* n_weights,
+                            float* weights, int* offsets, int* rows, int* cols);
+
 /* Write a kernel set in the reference's text format (the format
  * lpa_load_kernels reads): "D S", then per kernel "rows cols" and its rows
  * of "%f" weights.  Errors: LPA_EINVAL, LPA_EIO. */

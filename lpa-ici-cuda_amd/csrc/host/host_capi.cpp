@@ -63,6 +63,23 @@ lpa_status lpa_wedge_kernels(int num_dirs, int num_scales, const int *h, int *n_
     return LPA_OK;
 }
 
+lpa_status lpa_make_kernels(int num_dirs, int num_scales, const int *h, int order, int window, int *n_weights,
+                            float *weights, int *offsets, int *rows, int *cols)
+{
+    if (!n_weights) return lpa::fail(LPA_EINVAL, "null argument");
+    std::vector<float> w;
+    std::vector<int> off, r, c;
+    std::string err = lpa::lpa_kernels(num_dirs, num_scales, h, order, window, w, off, r, c);
+    if (!err.empty()) return lpa::fail(LPA_EINVAL, err);
+    *n_weights = (int)w.size();
+    if (weights) std::memcpy(weights, w.data(), w.size() * sizeof(float));
+    if (offsets) std::memcpy(offsets, off.data(), off.size() * sizeof(int));
+    if (rows) std::memcpy(rows, r.data(), r.size() * sizeof(int));
+    if (cols) std::memcpy(cols, c.data(), c.size() * sizeof(int));
+    lpa::clear_error();
+    return LPA_OK;
+}
+
 lpa_status lpa_write_kernels(const char *path, int num_dirs, int num_scales, const float *weights,
                              const int *offsets, const int *rows, const int *cols)
 {

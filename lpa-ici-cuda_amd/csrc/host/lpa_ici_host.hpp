@@ -81,8 +81,13 @@ bool write_hdr(const std::string &path, int width, int height, int channels, con
 // Deterministic synthetic image in [0,1].
 void synth_image(float *out, int width, int height, unsigned seed);
 
-// Uniform wedge kernel set (kernel_gen.cpp): D directions, S window lengths
-// h[0..S); dense (2h-1)^2 kernels in the loader's layout.  "" or an error.
+// Directional LPA kernel set (kernel_gen.cpp): D directions, S window
+// lengths h[0..S), polynomial order m, window 0 uniform / 1 Gaussian / 2
+// square; dense (2h-1)^2 kernels in the loader's layout.  "" or an error.
+std::string lpa_kernels(int D, int S, const int *h, int m, int window, std::vector<float> &w, std::vector<int> &off,
+                        std::vector<int> &rows, std::vector<int> &cols);
+
+// Uniform wedge kernel set: lpa_kernels with m = 0 and the uniform window.
 std::string wedge_kernels(int D, int S, const int *h, std::vector<float> &w, std::vector<int> &off,
                           std::vector<int> &rows, std::vector<int> &cols);
 

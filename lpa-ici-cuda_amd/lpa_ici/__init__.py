@@ -16,7 +16,7 @@ from ._lib import (DEFAULT_KERNELS, FWD_EXACT, FWD_GRAPH, FWD_PLAIN, FWD_RING_FL
 __all__ = [
     "DEFAULT_KERNELS", "GAUSSIAN_M20_KERNELS", "KERNEL_DIR", "LIB_PATH", "FWD_PLAIN", "FWD_EXACT", "FWD_GRAPH",
     "FWD_WEIGHTED", "FWD_RING_FLOAT",
-    "LpaError", "KernelSet", "Plan", "load_kernels", "wedge_kernels", "write_kernels", "kernel_variances",
+    "LpaError", "KernelSet", "Plan", "load_kernels", "wedge_kernels", "make_kernels", "write_kernels", "kernel_variances",
     "make_noisy", "psnr", "synth_image",
     "load_image", "write_hdr", "denoise", "estimates", "backward", "image_struct", "same_layout", "tile_configs", "lib",
     "workspace_peak",
@@ -164,6 +164,27 @@ def wedge_kernels(num_dirs, h):
     off, rows, cols = (np.zeros(K, np.int32) for _ in range(3))
     check(lib().lpa_wedge_kernels(int(num_dirs), hs.size, _ptr(hs), ctypes.byref(nw), _ptr(w), _ptr(off),
                                   _ptr(rows), _ptr(cols)))
+    return KernelSet(num_dirs, hs.size, w, off, rows, cols)
+
+
+WINDOWS = {"uniform": 0, "gaussian": 1, "square": 2}
+
+
+def make_kernels(num_dirs, h, order=1, window="gaussian"):
+    """Directional LPA kernel set: num_dirs directions, window lengths h (one
+    scale each), polynomial order `order` along the direction, window
+    "uniform", "gaussian" or "square"; see lpa_make_kernels in
+    include/lpa_ici.h."""
+    hs = _i32(h)
+    nw = ctypes.c_int()
+    wi = WINDOWS[window]
+    check(lib().lpa_make_kernels(int(num_dirs), hs.size, _ptr(hs), int(order), wi, ctypes.byref(nw), None, None,
+                                 None, None))
+    K = int(num_dirs) * hs.size
+    w = np.zeros(nw.value, np.float32)
+    off, rows, cols = (np.zeros(K, np.int32) for _ in range(3))
+    check(lib().lpa_make_kernels(int(num_dirs), hs.size, _ptr(hs), int(order), wi, ctypes.byref(nw), _ptr(w),
+                                 _ptr(off), _ptr(rows), _ptr(cols)))
     return KernelSet(num_dirs, hs.size, w, off, rows, cols)
 
 

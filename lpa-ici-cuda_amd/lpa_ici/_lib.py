@@ -85,6 +85,7 @@ SIGNATURES = {
                               _P, _P, _P, _P]),
     "lpa_kernel_variances": (None, [_I, _P, _P, _P, _P, _F, _P]),
     "lpa_wedge_kernels": (_I, [_I, _I, _P, ctypes.POINTER(_I), _P, _P, _P, _P]),
+    "lpa_make_kernels": (_I, [_I, _I, _P, _I, _I, ctypes.POINTER(_I), _P, _P, _P, _P]),
     "lpa_write_kernels": (_I, [ctypes.c_char_p, _I, _I, _P, _P, _P, _P]),
     "lpa_make_noisy": (None, [_P, _P, _LL, _F, ctypes.c_uint]),
     "lpa_psnr": (ctypes.c_double, [_P, _P, _LL]),

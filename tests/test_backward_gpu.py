@@ -120,6 +120,16 @@ def test_adjoint_shipped_sets(name):
     assert e_dx < 1e-4 and e_dk < 1e-4
 
 
+@pytest.mark.parametrize("D,h,m,window", [(16, [1, 2, 4, 8, 16, 24, 32], 2, "gaussian"), (4, [1, 3, 6, 11], 1, "square")])
+def test_adjoint_generated_lpa_sets(D, h, m, window):
+    """Generated LPA sets of order m (negative taps, no ring form) against the
+    FP64 adjoint."""
+    g = lpa_ici.make_kernels(D, h, m, window)
+    e_dx, e_dk = adjoint_error(g, noisy_synth(120, 100, seed=D + m))
+    print(f"generated D={D} m={m} {window} vs FP64: dx rel {e_dx:.2e}, dk rel {e_dk:.2e}")
+    assert e_dx < 1e-4 and e_dk < 1e-4
+
+
 @pytest.mark.parametrize("w,h", [(70, 45), (131, 9)])
 def test_adjoint_ragged_width(ks, w, h):
     """Widths that are not a multiple of 8: the first pass reads the scale

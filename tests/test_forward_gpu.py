@@ -152,6 +152,23 @@ def test_generated_wedge_sets(D, h):
     assert np.array_equal(y.view(np.uint32), yq.view(np.uint32))
 
 
+@pytest.mark.parametrize("D,h,m,window", [(16, [1, 2, 4, 8, 16, 24, 32], 2, "gaussian"), (8, [1, 3, 7, 15], 1, "uniform"),
+                                         (4, [1, 2, 5, 12], 2, "square")])
+def test_generated_lpa_sets(D, h, m, window):
+    """Generated LPA sets of order m (weighted, no ring form): exact mode
+    equals the oracle's v0_0 bits, the default weighted walk meets the bar."""
+    g = lpa_ici.make_kernels(D, h, m, window)
+    assert g.info()["ring_form"] == 0
+    clean = lpa_ici.synth_image(160, 96, 5)
+    x = lpa_ici.make_noisy(clean, SIGMA, SEED)
+    y, maps = run(g, x, flags=lpa_ici.FWD_EXACT)
+    yo, mo, _ = oracle.walk_f32(x, g, g.variances(SIGMA), GAMMA)
+    assert np.array_equal(maps, mo)
+    assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
+    y, maps = run(g, x)
+    check_fast_parity(g, x, clean, y, maps, f"generated D={D} m={m} {window}")
+
+
 @pytest.mark.parametrize("flags,label", [(0, "fixed point"), (lpa_ici.FWD_EXACT, "exact"),
                                          (lpa_ici.FWD_RING_FLOAT, "float ring"), (lpa_ici.FWD_WEIGHTED, "weighted")])
 def test_forward_keeps_estimates(ks, flags, label):
