@@ -75,6 +75,7 @@ struct lpa_kset {
                              d_ring_runs_flip,
                              d_ring_kern,
                              d_ring_scale,
+                             d_ring_scale ? d_ring_scale + K : nullptr,
                              std::ldexp(1.0f, ring_frac_bits),
                              std::ldexp(1.0f, -ring_frac_bits),
                              ring_range};
@@ -347,14 +348,18 @@ lpa_status ensure_uploaded(lpa_kset *ks)
             e = hipMemcpy(ks->d_ring_runs_flip, ks->ring_runs_flip.data(), sizeof(int4) * ks->ring_runs_flip.size(),
                           hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMalloc(&ks->d_ring_kern, sizeof(int2) * K);
-        if (e == hipSuccess) e = hipMalloc(&ks->d_ring_scale, sizeof(float) * K);
+        // [2][K]: w_s, then w_s * 2^-F (the fixed-point walk's factor)
+        if (e == hipSuccess) e = hipMalloc(&ks->d_ring_scale, sizeof(float) * 2 * K);
         if (e == hipSuccess && !ks->ring_runs.empty())
             e = hipMemcpy(ks->d_ring_runs, ks->ring_runs.data(), sizeof(int4) * ks->ring_runs.size(),
                           hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(ks->d_ring_kern, ks->ring_kern.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(ks->d_ring_scale, ks->ring_scale.data(), sizeof(float) * K, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            std::vector<float> scales(ks->ring_scale);
+            for (int k = 0; k < K; ++k) scales.push_back(std::ldexp(ks->ring_scale[k], -ks->ring_frac_bits));
+            e = hipMemcpy(ks->d_ring_scale, scales.data(), sizeof(float) * 2 * K, hipMemcpyHostToDevice);
+        }
     }
     if (e != hipSuccess) return hip_fail(e, "kernel set upload");
     ks->uploaded = true;
@@ -453,7 +458,7 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
     std::memcpy(host.data() + off_tiles, tiles.data(), sizeof(uint2) * tiles.size());
     HIP_TRY(hipMemcpyAsync(base, host.data(), bytes, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));  // `host` goes out of scope on return
-    const int bpc = lpa::blocks_per_cu(p->shape, ks->radius, ks->K);
+    const int bpc = lpa::blocks_per_cu(p->shape, ks->radius, lpa::inv_table_floats(ks->D, ks->S));
     p->grid = std::max(1, std::min(p->num_tiles, num_cus() * bpc));
     return LPA_OK;
 }
@@ -681,7 +686,8 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
         for (int y = 0; y < ty; ++y)
             for (int x = 0; x < tx; ++x) tiles.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
     }
-    const int grid = std::max(1, std::min((int)tiles.size(), num_cus() * lpa::blocks_per_cu(SH, ks->radius, ks->K)));
+    const int bpc = lpa::blocks_per_cu(SH, ks->radius, lpa::inv_table_floats(ks->D, ks->S));
+    const int grid = std::max(1, std::min((int)tiles.size(), num_cus() * bpc));
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t off_desc = 256;
     const size_t off_tiles = off_desc + up(sizeof(lpa::EstDesc) * n);

@@ -16,11 +16,43 @@
 // every direction and the tile is filled once.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lpa_device.hpp"
 #include "lpa_tile.hpp"
 
 namespace lpa {
 namespace {
+
+// The P values of a lane at consecutive addresses: as P / 4 words of four
+// elements when `whole` (every pixel inside, the row start aligned to four
+// elements), else element by element for the pixels inside the image.
+typedef float float4_words __attribute__((ext_vector_type(4)));
+
+template <int P, typename T, typename G>
+__device__ __forceinline__ void store_row(G *dst, const T (&v)[P], const bool (&inside)[P], bool whole)
+{
+    if constexpr (P % 4 == 0) {
+        if (whole) {
+            using Word = typename std::conditional<sizeof(T) == 1, uint32_t, float4_words>::type;
+            using GWord = __attribute__((address_space(1))) Word;
+#pragma unroll
+            for (int q = 0; q < P / 4; ++q) {
+                Word w;
+                if constexpr (sizeof(T) == 1)
+                    w = (uint32_t)v[4 * q] | (uint32_t)v[4 * q + 1] << 8 | (uint32_t)v[4 * q + 2] << 16 |
+                        (uint32_t)v[4 * q + 3] << 24;
+                else
+                    w = float4_words{v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+                reinterpret_cast<GWord *>(dst)[q] = w;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+        if (inside[p]) dst[p] = v[p];
+}
 
 // The scale walk of every direction for the P pixels of one lane, then the
 // aggregation and the stores.  `src` is the lane's LDS address: the float
@@ -46,6 +78,13 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
     }
     const size_t plane = (size_t)W * H;
     const size_t pix = (size_t)py * W + px0;
+    // Every pixel of the lane inside, and rows of the maps (and estimates)
+    // and of y starting on 4-element boundaries: the stores go out as whole
+    // 4-element words (px0 is a multiple of P, itself a multiple of 4).
+    const bool whole4 = P % 4 == 0 && inside[P - 1];
+    const bool map_aligned = W % 4 == 0 && ((uintptr_t)map & 3) == 0;
+    const bool est_aligned = W % 4 == 0 && ((uintptr_t)est & 15) == 0;
+    const bool y_aligned = pitch % 4 == 0 && ((uintptr_t)y & 15) == 0;
 
     for (int d = 0; d < ks.D; ++d) {
         float lo[P], hi[P], best[P];
@@ -56,20 +95,22 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
         for (int p = 0; p < P; ++p) {
             ring_sum[p] = 0.0f;
             ring_q[p] = 0u;
-            // Pixels outside the image start with an empty interval.
-            lo[p] = inside[p] ? -INFINITY : INFINITY;
-            hi[p] = inside[p] ? INFINITY : -INFINITY;
+            lo[p] = -INFINITY;
+            hi[p] = INFINITY;
             best[p] = 0.0f;
             sel[p] = -1;
         }
-        // alive: some pixel of the lane still has a non-empty interval (it
-        // is the `ok` of the previous scale; an empty intersection stays
-        // empty).  The wave stops walking this direction once no lane is.
+        // alive: some pixel of the lane inside the image still has a
+        // non-empty interval (it is the `ok` of the previous scale; an empty
+        // intersection stays empty).  The wave stops walking this direction
+        // once no lane is.  Pixels outside the image are walked along with
+        // the others but keep no lane alive, and nothing of theirs is stored.
         bool alive = false;
 #pragma unroll
         for (int p = 0; p < P; ++p) alive |= inside[p];
+        uint64_t live = __builtin_amdgcn_ballot_w64(alive);  // the wave's alive lanes
         for (int s = 0; s < S; ++s) {
-            if (!__any(alive)) break;
+            if (live == 0) break;
             const int k = d * S + s;
             float acc[P];
             if constexpr (MODE == kWalkRing) {
@@ -77,7 +118,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 // scale s), the previous scale's sum plus this scale's ring;
                 // one rounding for the product, none hidden in the interval.
                 if (alive) prefix_runs<P>(src, lp, rings.runs, load_const(rings.kern + k), ring_q);
-                const float wq = as_const(rings.scale)[k] * rings.qinv;
+                const float wq = as_const(rings.scale_q)[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {
@@ -104,36 +145,34 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                     best[p] = acc[p];
                     sel[p] = s;
                 }
-                any_ok |= ok;
+                any_ok |= ok && inside[p];
             }
             alive = any_ok;
+            live = __builtin_amdgcn_ballot_w64(any_ok);
         }
+        // 1/var of the selected scale, read once per direction (the
+        // intersection only shrinks, so the last scale with a non-empty
+        // interval is the selected one and the walk kept its index); the
+        // table's leading zero serves sel = -1
+        const float *__restrict__ inv_d = inv_lds + d * (S + 1) + 1;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            // 1/var of the selected scale, read once per direction (the
-            // intersection only shrinks, so the last scale with a non-empty
-            // interval is the selected one and the walk kept its index)
-            const float binv = sel[p] >= 0 ? inv_lds[d * S + sel[p]] : 0.0f;
+            const float binv = inv_d[sel[p]];
             num[p] = mul_add_rounded(num[p], best[p], binv);
             den[p] = den[p] + binv;
         }
         if (map) {
-            const auto gmap = as_global(map) + (size_t)d * plane + pix;
+            uint8_t v[P];
 #pragma unroll
-            for (int p = 0; p < P; ++p)
-                if (inside[p]) gmap[p] = (uint8_t)max(sel[p], 0);
+            for (int p = 0; p < P; ++p) v[p] = (uint8_t)max(sel[p], 0);
+            store_row<P>(as_global(map) + (size_t)d * plane + pix, v, inside, whole4 && map_aligned);
         }
-        if (est) {
-            const auto gest = as_global(est) + (size_t)d * plane + pix;
-#pragma unroll
-            for (int p = 0; p < P; ++p)
-                if (inside[p]) gest[p] = best[p];
-        }
+        if (est) store_row<P>(as_global(est) + (size_t)d * plane + pix, best, inside, whole4 && est_aligned);
     }
-    const auto yrow = as_global(y) + (size_t)py * pitch + px0;
+    float out[P];
 #pragma unroll
-    for (int p = 0; p < P; ++p)
-        if (inside[p]) yrow[p] = num[p] / den[p];
+    for (int p = 0; p < P; ++p) out[p] = num[p] / den[p];
+    store_row<P>(as_global(y) + (size_t)py * pitch + px0, out, inside, whole4 && y_aligned);
 }
 
 // The lane's cell of a tile: column tx (in units of P pixels) and row ty.
@@ -174,10 +213,14 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
     // boundary of a run that starts at that pixel.
     const int lane_off = (ty + R) * lp + tx * P + R;
     // 1/var per kernel, after the window: read once per direction at the
-    // selected scale (the staging barrier orders these writes)
-    const int K = ks.D * ks.S;
+    // selected scale; per direction a zero, then the S scales (the staging
+    // barrier orders these writes)
+    const int S = ks.S, K = ks.D * S;
     float *inv_lds = tile + lds_floats(TW + 1, TH, R, PM);
-    for (int i = tid; i < K; i += NT) inv_lds[i] = as_const(tab)[2 * K + i];
+    for (int i = tid; i < inv_table_floats(ks.D, S); i += NT) {
+        const int d = i / (S + 1), j = i - d * (S + 1);
+        inv_lds[i] = j == 0 ? 0.0f : as_const(tab)[2 * K + d * S + j - 1];
+    }
     if constexpr (MODE == kWalkRing) {
         uint32_t *q = (uint32_t *)tile;
         if (stage_prefix<TW, TH, NT>(q, lp, x, W, H, pitch, x0, y0, R, rings.qscale, rings.qrange)) {
@@ -242,7 +285,7 @@ __device__ __forceinline__ void estimate_lane(const T *__restrict__ src, int lp,
             float acc[P];
             if constexpr (MODE == kWalkRing) {
                 if (need) prefix_runs<P>(src, lp, rings.runs, load_const(rings.kern + k), ring_q);
-                const float wq = as_const(rings.scale)[k] * rings.qinv;
+                const float wq = as_const(rings.scale_q)[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {
@@ -362,10 +405,10 @@ void fwd_batched_kernel(DevKset ks, DevRings rings, const ImgDesc *__restrict__ 
 __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 
 // LDS of one work-group: the window (with its leading column) and the
-// [K] table of 1/var.
-size_t fwd_lds_bytes(const TileShape &sh, int R, int K)
+// table of 1/var (table_floats = inv_table_floats(D, S) words).
+size_t fwd_lds_bytes(const TileShape &sh, int R, int table_floats)
 {
-    return sizeof(float) * ((size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm) + (size_t)K);
+    return sizeof(float) * ((size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm) + (size_t)table_floats);
 }
 
 // Launches that need more LDS than the default limit raise it first.
@@ -385,7 +428,7 @@ hipError_t plain_launch(const DevKset &ks, const DevRings &rings, const ImgDesc 
     const int tiles_y = (img.H + sh.th - 1) / sh.th;
     const int num_tiles = img.tiles_x * tiles_y;
     const int per_xcd = (num_tiles + 7) / 8;
-    const size_t lds = fwd_lds_bytes(sh, R, ks.D * ks.S);
+    const size_t lds = fwd_lds_bytes(sh, R, inv_table_floats(ks.D, ks.S));
     const hipError_t e = allow_lds(fwd_plain_kernel<SH, MODE>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((fwd_plain_kernel<SH, MODE>), dim3(8 * per_xcd), dim3(sh.nt), lds, stream, ks, rings, img, R,
@@ -410,7 +453,7 @@ hipError_t batched_launch(const DevKset &ks, const DevRings &rings, const ImgDes
                           int num_tiles, int *counter, int R, int grid, hipStream_t stream)
 {
     constexpr TileShape sh = kShapes[SH];
-    const size_t lds = fwd_lds_bytes(sh, R, ks.D * ks.S);
+    const size_t lds = fwd_lds_bytes(sh, R, inv_table_floats(ks.D, ks.S));
     const hipError_t e = allow_lds(fwd_batched_kernel<SH, MODE>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((fwd_batched_kernel<SH, MODE>), dim3(grid), dim3(sh.nt), lds, stream, ks, rings, imgs, tiles,
@@ -433,9 +476,9 @@ hipError_t batched_dispatch(WalkMode mode, const DevKset &ks, const DevRings &ri
 }
 
 template <int SH>
-int occupancy(int R, int K)
+int occupancy(int R, int table_floats)
 {
-    const size_t lds = fwd_lds_bytes(kShapes[SH], R, K);
+    const size_t lds = fwd_lds_bytes(kShapes[SH], R, table_floats);
     if (allow_lds(fwd_batched_kernel<SH, kWalkRing>, lds) != hipSuccess) return 1;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, kWalkRing>, kShapes[SH].nt, lds) !=
@@ -517,19 +560,19 @@ hipError_t launch_estimates(WalkMode mode, int shape, const DevKset &ks, const D
     }
 }
 
-int blocks_per_cu(int shape, int radius, int num_kernels)
+int blocks_per_cu(int shape, int radius, int table_floats)
 {
     switch (shape) {
-    case 0: return occupancy<0>(radius, num_kernels);
-    case 1: return occupancy<1>(radius, num_kernels);
-    case 2: return occupancy<2>(radius, num_kernels);
-    case 3: return occupancy<3>(radius, num_kernels);
-    case 4: return occupancy<4>(radius, num_kernels);
-    case 5: return occupancy<5>(radius, num_kernels);
-    case 6: return occupancy<6>(radius, num_kernels);
-    case 7: return occupancy<7>(radius, num_kernels);
-    case 8: return occupancy<8>(radius, num_kernels);
-    case 9: return occupancy<9>(radius, num_kernels);
+    case 0: return occupancy<0>(radius, table_floats);
+    case 1: return occupancy<1>(radius, table_floats);
+    case 2: return occupancy<2>(radius, table_floats);
+    case 3: return occupancy<3>(radius, table_floats);
+    case 4: return occupancy<4>(radius, table_floats);
+    case 5: return occupancy<5>(radius, table_floats);
+    case 6: return occupancy<6>(radius, table_floats);
+    case 7: return occupancy<7>(radius, table_floats);
+    case 8: return occupancy<8>(radius, table_floats);
+    case 9: return occupancy<9>(radius, table_floats);
     default: return 1;
     }
 }

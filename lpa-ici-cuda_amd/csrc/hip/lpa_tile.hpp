@@ -335,12 +335,20 @@ __device__ __forceinline__ bool stage_prefix(uint32_t *__restrict__ q, int lp, c
     return block_vote<NT, true>(fits);
 }
 
+// a + b + c in one v_add3_u32 (wrapping, as the fixed-point sums are).
+__device__ __forceinline__ uint32_t add3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_add3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // rs[p] += fixed-point sum over the runs kr of the tile, for the P pixels
 // whose left-boundary prefix values start at `lane` (prefix table as written
 // by stage_prefix): a run (row dr, column c0, length L) adds
 // Q[dr][c0 + L + p] - Q[dr][c0 + p], whatever its length.  Runs go in pairs:
 // one scalar load brings both records, the four windows of P prefix values
-// are read together (4P reads in flight), and the pair costs four integer
+// are read together (4P reads in flight), and the pair costs three integer
 // operations per pixel.  There is no branch on the run length.
 template <int P>
 __device__ __forceinline__ void prefix_runs(const uint32_t *__restrict__ lane, int lp, const int4 *__restrict__ runs,
@@ -363,7 +371,7 @@ __device__ __forceinline__ void prefix_runs(const uint32_t *__restrict__ lane, i
             b1[p] = vb[p];
         }
 #pragma unroll
-        for (int p = 0; p < P; ++p) rs[p] = (rs[p] + b0[p] + b1[p]) - (a0[p] + a1[p]);
+        for (int p = 0; p < P; ++p) rs[p] = add3(rs[p], b0[p], b1[p]) - a0[p] - a1[p];
     }
     if (r < kr.y) {
         const int4 u = load_const(runs + kr.x + r);
