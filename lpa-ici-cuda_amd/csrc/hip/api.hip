@@ -458,7 +458,7 @@ lpa_status plan_build(lpa_kset *ks, const lpa_image *imgs, int n, unsigned flags
     std::memcpy(host.data() + off_tiles, tiles.data(), sizeof(uint2) * tiles.size());
     HIP_TRY(hipMemcpyAsync(base, host.data(), bytes, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));  // `host` goes out of scope on return
-    const int bpc = lpa::blocks_per_cu(p->shape, ks->radius, lpa::inv_table_floats(ks->D, ks->S));
+    const int bpc = lpa::blocks_per_cu(p->shape, ks->radius, ks->K);
     p->grid = std::max(1, std::min(p->num_tiles, num_cus() * bpc));
     return LPA_OK;
 }
@@ -686,7 +686,7 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
         for (int y = 0; y < ty; ++y)
             for (int x = 0; x < tx; ++x) tiles.push_back(uint2{(unsigned)i, ((unsigned)y << 16) | (unsigned)x});
     }
-    const int bpc = lpa::blocks_per_cu(SH, ks->radius, lpa::inv_table_floats(ks->D, ks->S));
+    const int bpc = lpa::blocks_per_cu(SH, ks->radius, ks->K);
     const int grid = std::max(1, std::min((int)tiles.size(), num_cus() * bpc));
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t off_desc = 256;

@@ -96,13 +96,6 @@ __host__ __device__ constexpr int lds_floats(int tw, int th, int r, int pm = 1)
 {
     return (th + 2 * r) * lds_pitch(tw, r, pm);
 }
-// The forward's LDS table of 1/var after the window: per direction a zero
-// and then the S scales, so that index 1 + s with s = -1 (no scale selected)
-// reads 0 without a branch.
-__host__ __device__ constexpr int inv_table_floats(int num_dirs, int num_scales)
-{
-    return num_dirs * (num_scales + 1);
-}
 
 // One image of an adjoint batch: the forward's image (x, y = the forward's
 // output, map = its scale indices, tab) plus the upstream gradient g, the
@@ -142,9 +135,8 @@ hipError_t launch_forward_plain(int shape, WalkMode mode, const DevKset &ks, con
 hipError_t launch_forward_batched(int shape, WalkMode mode, const DevKset &ks, const DevRings &rings,
                                   const ImgDesc *d_imgs, const uint2 *d_tiles, int num_tiles, int *d_counter,
                                   int radius, int grid, hipStream_t stream);
-// Work-groups of one shape that fit on a CU at this radius, with
-// table_floats words of LDS after the window (inv_table_floats).
-int blocks_per_cu(int shape, int radius, int table_floats);
+// Work-groups of one shape that fit on a CU at this radius and kernel count.
+int blocks_per_cu(int shape, int radius, int num_kernels);
 
 // Per-direction estimates at frozen scales (lpa_fwd.hip): est = [D][H][W]
 // floats per image, dense like the scale map; a persistent grid over the tile

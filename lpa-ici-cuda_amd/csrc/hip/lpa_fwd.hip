@@ -86,30 +86,59 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
     const bool est_aligned = W % 4 == 0 && ((uintptr_t)est & 15) == 0;
     const bool y_aligned = pitch % 4 == 0 && ((uintptr_t)y & 15) == 0;
 
+    // A wave wholly outside the image has nothing to walk or store.
+    bool any_inside = false;
+#pragma unroll
+    for (int p = 0; p < P; ++p) any_inside |= inside[p];
+    if (__builtin_amdgcn_ballot_w64(any_inside) == 0) return;
+
     for (int d = 0; d < ks.D; ++d) {
         float lo[P], hi[P], best[P];
-        int sel[P];  // -1 until a scale's interval is non-empty
+        int sel[P];           // the last scale whose interval is non-empty
         float ring_sum[P];    // float ring sums: running sum over scales 0..s
         uint32_t ring_q[P];   // fixed-point ring sums
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             ring_sum[p] = 0.0f;
             ring_q[p] = 0u;
-            lo[p] = -INFINITY;
-            hi[p] = INFINITY;
-            best[p] = 0.0f;
-            sel[p] = -1;
+        }
+        // Scale 0: the interval [yhat - thr, yhat + thr] with thr >= 0 is
+        // never empty (a NaN bound leaves the infinite start in place, as
+        // the reference's std::max / std::min do), so every pixel selects
+        // it.  The fixed-point walk takes it without the test; the other
+        // walks (whose estimate code is long) run it through the loop.
+        constexpr int PEEL = MODE == kWalkRing ? 1 : 0;
+        if constexpr (PEEL) {
+            // yhat_0 = (w_0 2^-F) * (fixed-point sum over the support of
+            // scale 0); a lane with no pixel inside the image skips the reads
+            if (any_inside) prefix_runs<P>(src, lp, rings.runs, load_const(rings.kern + d * S), ring_q);
+            const float wq = as_const(rings.scale_q)[d * S];
+            const float thr = as_const(thr_t)[d * S];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float acc = mul_rounded((float)(int)ring_q[p], wq);
+                lo[p] = max_f32(-INFINITY, acc - thr);
+                hi[p] = min_f32(INFINITY, acc + thr);
+                best[p] = acc;
+                sel[p] = 0;
+            }
+        } else {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                lo[p] = -INFINITY;
+                hi[p] = INFINITY;
+                best[p] = 0.0f;
+                sel[p] = 0;
+            }
         }
         // alive: some pixel of the lane inside the image still has a
         // non-empty interval (it is the `ok` of the previous scale; an empty
         // intersection stays empty).  The wave stops walking this direction
         // once no lane is.  Pixels outside the image are walked along with
         // the others but keep no lane alive, and nothing of theirs is stored.
-        bool alive = false;
-#pragma unroll
-        for (int p = 0; p < P; ++p) alive |= inside[p];
+        bool alive = any_inside;
         uint64_t live = __builtin_amdgcn_ballot_w64(alive);  // the wave's alive lanes
-        for (int s = 0; s < S; ++s) {
+        for (int s = PEEL; s < S; ++s) {
             if (live == 0) break;
             const int k = d * S + s;
             float acc[P];
@@ -123,7 +152,9 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {
                 // yhat_s = w_s * (float sum over the support of scale s).
-                if (alive) correlate_runs<P, false, 1, false>(src, lp, rings.runs, nullptr, load_const(rings.kern + k), ring_sum);
+                if (alive)
+                    correlate_runs<P, false, 1, false>(src, lp, rings.runs, nullptr, load_const(rings.kern + k),
+                                                       ring_sum);
                 const float w = as_const(rings.scale)[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;
@@ -152,19 +183,17 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
         }
         // 1/var of the selected scale, read once per direction (the
         // intersection only shrinks, so the last scale with a non-empty
-        // interval is the selected one and the walk kept its index); the
-        // table's leading zero serves sel = -1
-        const float *__restrict__ inv_d = inv_lds + d * (S + 1) + 1;
+        // interval is the selected one and the walk kept its index)
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const float binv = inv_d[sel[p]];
+            const float binv = inv_lds[d * S + sel[p]];
             num[p] = mul_add_rounded(num[p], best[p], binv);
             den[p] = den[p] + binv;
         }
         if (map) {
             uint8_t v[P];
 #pragma unroll
-            for (int p = 0; p < P; ++p) v[p] = (uint8_t)max(sel[p], 0);
+            for (int p = 0; p < P; ++p) v[p] = (uint8_t)sel[p];
             store_row<P>(as_global(map) + (size_t)d * plane + pix, v, inside, whole4 && map_aligned);
         }
         if (est) store_row<P>(as_global(est) + (size_t)d * plane + pix, best, inside, whole4 && est_aligned);
@@ -213,14 +242,10 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
     // boundary of a run that starts at that pixel.
     const int lane_off = (ty + R) * lp + tx * P + R;
     // 1/var per kernel, after the window: read once per direction at the
-    // selected scale; per direction a zero, then the S scales (the staging
-    // barrier orders these writes)
-    const int S = ks.S, K = ks.D * S;
+    // selected scale (the staging barrier orders these writes)
+    const int K = ks.D * ks.S;
     float *inv_lds = tile + lds_floats(TW + 1, TH, R, PM);
-    for (int i = tid; i < inv_table_floats(ks.D, S); i += NT) {
-        const int d = i / (S + 1), j = i - d * (S + 1);
-        inv_lds[i] = j == 0 ? 0.0f : as_const(tab)[2 * K + d * S + j - 1];
-    }
+    for (int i = tid; i < K; i += NT) inv_lds[i] = as_const(tab)[2 * K + i];
     if constexpr (MODE == kWalkRing) {
         uint32_t *q = (uint32_t *)tile;
         if (stage_prefix<TW, TH, NT>(q, lp, x, W, H, pitch, x0, y0, R, rings.qscale, rings.qrange)) {
@@ -405,10 +430,10 @@ void fwd_batched_kernel(DevKset ks, DevRings rings, const ImgDesc *__restrict__ 
 __global__ void reset_counter_kernel(int *counter) { *counter = 0; }
 
 // LDS of one work-group: the window (with its leading column) and the
-// table of 1/var (table_floats = inv_table_floats(D, S) words).
-size_t fwd_lds_bytes(const TileShape &sh, int R, int table_floats)
+// [K] table of 1/var.
+size_t fwd_lds_bytes(const TileShape &sh, int R, int K)
 {
-    return sizeof(float) * ((size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm) + (size_t)table_floats);
+    return sizeof(float) * ((size_t)lds_floats(sh.tw + 1, sh.th, R, sh.pm) + (size_t)K);
 }
 
 // Launches that need more LDS than the default limit raise it first.
@@ -428,7 +453,7 @@ hipError_t plain_launch(const DevKset &ks, const DevRings &rings, const ImgDesc 
     const int tiles_y = (img.H + sh.th - 1) / sh.th;
     const int num_tiles = img.tiles_x * tiles_y;
     const int per_xcd = (num_tiles + 7) / 8;
-    const size_t lds = fwd_lds_bytes(sh, R, inv_table_floats(ks.D, ks.S));
+    const size_t lds = fwd_lds_bytes(sh, R, ks.D * ks.S);
     const hipError_t e = allow_lds(fwd_plain_kernel<SH, MODE>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((fwd_plain_kernel<SH, MODE>), dim3(8 * per_xcd), dim3(sh.nt), lds, stream, ks, rings, img, R,
@@ -453,7 +478,7 @@ hipError_t batched_launch(const DevKset &ks, const DevRings &rings, const ImgDes
                           int num_tiles, int *counter, int R, int grid, hipStream_t stream)
 {
     constexpr TileShape sh = kShapes[SH];
-    const size_t lds = fwd_lds_bytes(sh, R, inv_table_floats(ks.D, ks.S));
+    const size_t lds = fwd_lds_bytes(sh, R, ks.D * ks.S);
     const hipError_t e = allow_lds(fwd_batched_kernel<SH, MODE>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((fwd_batched_kernel<SH, MODE>), dim3(grid), dim3(sh.nt), lds, stream, ks, rings, imgs, tiles,
@@ -476,9 +501,9 @@ hipError_t batched_dispatch(WalkMode mode, const DevKset &ks, const DevRings &ri
 }
 
 template <int SH>
-int occupancy(int R, int table_floats)
+int occupancy(int R, int K)
 {
-    const size_t lds = fwd_lds_bytes(kShapes[SH], R, table_floats);
+    const size_t lds = fwd_lds_bytes(kShapes[SH], R, K);
     if (allow_lds(fwd_batched_kernel<SH, kWalkRing>, lds) != hipSuccess) return 1;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fwd_batched_kernel<SH, kWalkRing>, kShapes[SH].nt, lds) !=
@@ -560,19 +585,19 @@ hipError_t launch_estimates(WalkMode mode, int shape, const DevKset &ks, const D
     }
 }
 
-int blocks_per_cu(int shape, int radius, int table_floats)
+int blocks_per_cu(int shape, int radius, int num_kernels)
 {
     switch (shape) {
-    case 0: return occupancy<0>(radius, table_floats);
-    case 1: return occupancy<1>(radius, table_floats);
-    case 2: return occupancy<2>(radius, table_floats);
-    case 3: return occupancy<3>(radius, table_floats);
-    case 4: return occupancy<4>(radius, table_floats);
-    case 5: return occupancy<5>(radius, table_floats);
-    case 6: return occupancy<6>(radius, table_floats);
-    case 7: return occupancy<7>(radius, table_floats);
-    case 8: return occupancy<8>(radius, table_floats);
-    case 9: return occupancy<9>(radius, table_floats);
+    case 0: return occupancy<0>(radius, num_kernels);
+    case 1: return occupancy<1>(radius, num_kernels);
+    case 2: return occupancy<2>(radius, num_kernels);
+    case 3: return occupancy<3>(radius, num_kernels);
+    case 4: return occupancy<4>(radius, num_kernels);
+    case 5: return occupancy<5>(radius, num_kernels);
+    case 6: return occupancy<6>(radius, num_kernels);
+    case 7: return occupancy<7>(radius, num_kernels);
+    case 8: return occupancy<8>(radius, num_kernels);
+    case 9: return occupancy<9>(radius, num_kernels);
     default: return 1;
     }
 }
