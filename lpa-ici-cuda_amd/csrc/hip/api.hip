@@ -77,7 +77,6 @@ struct lpa_kset {
                              d_ring_scale,
                              d_ring_scale ? d_ring_scale + K : nullptr,
                              std::ldexp(1.0f, ring_frac_bits),
-                             std::ldexp(1.0f, -ring_frac_bits),
                              ring_range};
     }
     lpa::DevAdjoint adjoint() const

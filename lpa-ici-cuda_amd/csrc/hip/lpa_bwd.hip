@@ -427,7 +427,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                         const int k = d * S + s;
                         if (fixed) {
                             if (need) prefix_runs<GP>(lane_q, lpq, rings.runs, load_const(rings.kern + k), rq);
-                            const float wq = as_const(rings.scale)[k] * rings.qinv;
+                            const float wq = as_const(rings.scale_q)[k];  // w_s 2^-F, as the forward
 #pragma unroll
                             for (int p = 0; p < GP; ++p)
                                 if (sd[p] == s) yh[p] = mul_rounded((float)(int)rq[p], wq);

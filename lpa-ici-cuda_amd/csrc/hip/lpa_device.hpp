@@ -35,7 +35,6 @@ struct DevRings {
     const float *scale;    // w_s per kernel
     const float *scale_q;  // w_s * 2^-F per kernel (exact: a power-of-two scaling)
     float qscale;  // 2^F: fixed-point scale of tile values for ring sums
-    float qinv;    // 2^-F
     float qrange;  // windows whose values all lie strictly within +-qrange use fixed point
 };
 
