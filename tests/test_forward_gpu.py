@@ -12,6 +12,8 @@ Bars (the reference's algorithm in FP32 on the same seeded noisy image):
   * the fixed-point walk is also bit-identical to the oracle's restatement of
     it (integer sums do not depend on the order of the taps).
 """
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -273,6 +275,27 @@ def test_other_kernel_sets_exact(name):
     yo, mo, _ = oracle.walk_f32(x, ksx, ksx.variances(SIGMA), GAMMA)
     assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
     assert np.array_equal(maps, mo)
+
+
+SHIPPED = sorted(f[:-len(".gz")] for f in os.listdir(lpa_ici.KERNEL_DIR) if f.endswith(".txt.gz"))
+
+
+@pytest.mark.parametrize("name", SHIPPED)
+def test_every_shipped_set_both_modes(name):
+    """Every kernel set the reference ships (its kernels/ folder): the exact
+    mode bit-identical to the oracle's v0_0 walk (output and scale maps), and
+    the default walk (fixed-point ring sums for ring-form sets, FMA for the
+    others) at the bar."""
+    ksx = kernels(name)
+    x = noisy_synth(128, 96, seed=11)
+    clean = lpa_ici.synth_image(128, 96, 11)
+    y, maps = run(ksx, x, flags=lpa_ici.FWD_EXACT)
+    yo, mo, _ = oracle.walk_f32(x, ksx, ksx.variances(SIGMA), GAMMA)
+    assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
+    assert np.array_equal(maps, mo)
+    walk = "fixed point" if ksx.info()["ring_form"] else "weighted FMA"
+    yf, mf = run(ksx, x)
+    check_fast_parity(ksx, x, clean, yf, mf, f"{name} ({walk})")
 
 
 def test_batched_equals_plain(ks):
