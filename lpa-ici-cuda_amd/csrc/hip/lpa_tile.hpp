@@ -192,12 +192,51 @@ __device__ __forceinline__ void correlate_runs(const float *__restrict__ lane, i
     }
 }
 
-// Weighted correlation of kernel k of a set (see correlate_runs).
+// correlate_runs over a kernel's stretches of consecutive runs of one length
+// ({first run, number of runs, length}): one dispatch on the length per
+// stretch, then a loop over its runs with the unrolled body.  The taps are
+// added in the same order as run by run.  Each dispatch leaves the sums in
+// the registers the switch's cases share and copies them back to the loop's,
+// so the fewer dispatches, the fewer copies and scalar branches.
+template <int P, bool EXACT>
+__device__ __forceinline__ void correlate_groups(const float *__restrict__ lane, int lp,
+                                                 const int4 *__restrict__ runs, const float *__restrict__ weights,
+                                                 const int4 *__restrict__ groups, int2 kg, float (&acc)[P])
+{
+    for (int g = 0; g < kg.y; ++g) {
+        const int4 grp = load_const(groups + kg.x + g);
+        switch (grp.z) {
+#define LPA_GROUP_CASE(L)                                                                                          \
+    case L:                                                                                                        \
+        for (int r = 0; r < grp.y; ++r) {                                                                          \
+            const int4 run = load_const(runs + grp.x + r);                                                         \
+            const float *__restrict__ src = lane + __builtin_amdgcn_readfirstlane(run.x * lp + run.y);             \
+            run_taps<L, P, EXACT, true>(src, weights + run.w, acc);                                                \
+        }                                                                                                          \
+        break;
+            LPA_GROUP_CASE(1) LPA_GROUP_CASE(2) LPA_GROUP_CASE(3) LPA_GROUP_CASE(4) LPA_GROUP_CASE(5)
+            LPA_GROUP_CASE(6) LPA_GROUP_CASE(7) LPA_GROUP_CASE(8) LPA_GROUP_CASE(9) LPA_GROUP_CASE(10)
+            LPA_GROUP_CASE(11) LPA_GROUP_CASE(12) LPA_GROUP_CASE(13) LPA_GROUP_CASE(14) LPA_GROUP_CASE(15)
+            LPA_GROUP_CASE(16) LPA_GROUP_CASE(17) LPA_GROUP_CASE(18) LPA_GROUP_CASE(19) LPA_GROUP_CASE(20)
+            LPA_GROUP_CASE(21) LPA_GROUP_CASE(22) LPA_GROUP_CASE(23) LPA_GROUP_CASE(24) LPA_GROUP_CASE(25)
+            LPA_GROUP_CASE(26) LPA_GROUP_CASE(27) LPA_GROUP_CASE(28) LPA_GROUP_CASE(29) LPA_GROUP_CASE(30)
+            LPA_GROUP_CASE(31) LPA_GROUP_CASE(32)
+#undef LPA_GROUP_CASE
+        default: break;
+        }
+    }
+}
+
+// Weighted correlation of kernel k of a set (see correlate_runs): the
+// unrolled body goes by stretches of runs of one length.
 template <int P, bool EXACT, int BODY = 1>
 __device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp, const DevKset &ks, int k,
                                           float (&acc)[P])
 {
-    correlate_runs<P, EXACT, BODY, true>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
+    if constexpr (BODY == 1)
+        correlate_groups<P, EXACT>(lane, lp, ks.runs, ks.weights, ks.groups, load_const(ks.kern_groups + k), acc);
+    else
+        correlate_runs<P, EXACT, BODY, true>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
 }
 
 // Stage the (TW + 2R) x (TH + 2R) window whose top-left image pixel is
