@@ -77,11 +77,25 @@ struct lpa_kset {
     {
         return lpa::DevKset{d_runs, d_weights, d_kern, d_groups, d_kern_groups, D, S};
     }
-    lpa::DevRings rings() const
+    // The ring runs as byte-offset pairs, one table per prefix row pitch that
+    // a tile configuration or the adjoint uses (all built at upload).
+    struct RingPairs {
+        int lp;
+        int4 *d_pairs;
+        int2 *d_kern;
+    };
+    std::vector<RingPairs> ring_pairs;
+    // Ring tables for a prefix table of row pitch lp (pairs null if none was built).
+    lpa::DevRings rings(int lp) const
     {
+        const RingPairs *rp = nullptr;
+        for (const RingPairs &t : ring_pairs)
+            if (t.lp == lp) rp = &t;
         return lpa::DevRings{d_ring_runs,
                              d_ring_runs_flip,
                              d_ring_kern,
+                             rp ? rp->d_pairs : nullptr,
+                             rp ? rp->d_kern : nullptr,
                              d_ring_scale,
                              d_ring_scale ? d_ring_scale + K : nullptr,
                              std::ldexp(1.0f, ring_frac_bits),
@@ -385,6 +399,36 @@ lpa_status ensure_uploaded(lpa_kset *ks)
             for (int k = 0; k < K; ++k) scales.push_back(std::ldexp(ks->ring_scale[k], -ks->ring_frac_bits));
             e = hipMemcpy(ks->d_ring_scale, scales.data(), sizeof(float) * 2 * K, hipMemcpyHostToDevice);
         }
+        // Byte-offset pairs for every prefix row pitch in use: the forward's
+        // tile configurations and the adjoint's 64-wide tiles.
+        std::vector<int> pitches{lpa::prefix_pitch(lpa::kBwdTileW, ks->radius)};
+        for (int i = 0; i < lpa::kNumShapes; ++i) {
+            const int lp = lpa::prefix_pitch(lpa::kShapes[i].tw, ks->radius, lpa::kShapes[i].pm);
+            if (std::find(pitches.begin(), pitches.end(), lp) == pitches.end()) pitches.push_back(lp);
+        }
+        for (int lp : pitches) {
+            if (e != hipSuccess) break;
+            std::vector<int4> pairs;
+            std::vector<int2> pkern;
+            for (int k = 0; k < K; ++k) {
+                const int2 kr = ks->ring_kern[k];
+                pkern.push_back(int2{(int)pairs.size(), kr.y});
+                for (int r = kr.x; r < kr.x + kr.y; r += 2) {
+                    const int4 u = ks->ring_runs[r];
+                    const int4 v = r + 1 < kr.x + kr.y ? ks->ring_runs[r + 1] : u;
+                    const int au = 4 * (u.x * lp + u.y), av = 4 * (v.x * lp + v.y);
+                    pairs.push_back(int4{au, au + 4 * u.z, av, av + 4 * v.z});
+                }
+            }
+            if (pairs.empty()) pairs.push_back(int4{0, 0, 0, 0});
+            lpa_kset::RingPairs t{lp, nullptr, nullptr};
+            e = hipMalloc(&t.d_pairs, sizeof(int4) * pairs.size());
+            if (e == hipSuccess) e = hipMalloc(&t.d_kern, sizeof(int2) * K);
+            if (e == hipSuccess)
+                e = hipMemcpy(t.d_pairs, pairs.data(), sizeof(int4) * pairs.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(t.d_kern, pkern.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
+            ks->ring_pairs.push_back(t);
+        }
     }
     if (e != hipSuccess) return hip_fail(e, "kernel set upload");
     ks->uploaded = true;
@@ -522,7 +566,8 @@ lpa_status plan_launch(lpa_plan *p, hipStream_t stream)
 {
     const lpa::WalkMode mode = walk_mode(p->ks, p->flags);
     const lpa::DevKset dk = p->ks->dev();
-    const lpa::DevRings dr = p->ks->rings();
+    const lpa::TileShape &sh = lpa::kShapes[p->shape];
+    const lpa::DevRings dr = p->ks->rings(lpa::prefix_pitch(sh.tw, p->ks->radius, sh.pm));
     if (p->flags & LPA_FWD_PLAIN) {
         for (int i = 0; i < p->n; ++i) {
             hipError_t e = lpa::launch_forward_plain(p->shape, mode, dk, dr, p->host_desc[i], p->ks->radius, stream);
@@ -579,6 +624,10 @@ void lpa_kset_destroy(lpa_kset_t ks)
                     (void *)ks->d_tap_kernel, (void *)ks->d_kern_w, (void *)ks->d_sumsq, (void *)ks->d_ring_runs,
                     (void *)ks->d_ring_runs_flip, (void *)ks->d_ring_kern, (void *)ks->d_ring_scale})
         if (p) (void)hipFree(p);
+    for (const auto &t : ks->ring_pairs) {
+        if (t.d_pairs) (void)hipFree(t.d_pairs);
+        if (t.d_kern) (void)hipFree(t.d_kern);
+    }
     delete ks;
 }
 
@@ -741,7 +790,10 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
     hipError_t e = hipMemcpyAsync(mem, host.data(), bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
     if (e == hipSuccess)
-        e = lpa::launch_estimates(mode, SH, ks->dev(), ks->has_rings ? ks->rings() : lpa::DevRings{},
+        e = lpa::launch_estimates(mode, SH, ks->dev(),
+                                  ks->has_rings ? ks->rings(lpa::prefix_pitch(lpa::kShapes[SH].tw, ks->radius,
+                                                                                lpa::kShapes[SH].pm))
+                                                : lpa::DevRings{},
                                   (const lpa::EstDesc *)(base + off_desc), (const uint2 *)(base + off_tiles),
                                   (int)tiles.size(), (int *)base, ks->radius, grid, s);
     hipError_t ef = hipFreeAsync(mem, s);
@@ -826,7 +878,9 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     hipError_t e = hipMemcpyAsync(mem, host.data(), host_bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
     if (e == hipSuccess)
-        e = lpa::launch_backward(ks->dev(), ks->has_rings ? ks->rings() : lpa::DevRings{},
+        e = lpa::launch_backward(ks->dev(),
+                                 ks->has_rings ? ks->rings(lpa::prefix_pitch(lpa::kBwdTileW, ks->radius))
+                                               : lpa::DevRings{},
                                  ks->adjoint(), (const lpa::BwdDesc *)(base + off_desc),
                                  (const uint2 *)(base + off_tiles), (int)tiles.size(),
                                  (const uint2 *)(base + off_dxt), (int)dx_tiles.size(), (int *)base, ks->radius,

@@ -328,7 +328,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
     extern __shared__ __align__(16) float lds_f[];
     __shared__ int next;
     const int S = ks.S, D = ks.D, K = D * S;
-    const int lp = lds_pitch(GW, R, kBwdPitchResidue), lpq = lds_pitch(GW + 1, R);
+    const int lp = lds_pitch(GW, R, kBwdPitchResidue), lpq = prefix_pitch(GW, R);  // lpq: the ring pairs' pitch
     float *xt = lds_f;
     uint32_t *qt = (uint32_t *)lds_f;
     float *tab = lds_f + grad_tab_offset(R);
@@ -426,7 +426,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                         const bool need = (ls >> s) != 0u;
                         const int k = d * S + s;
                         if (fixed) {
-                            if (need) prefix_runs<GP>(lane_q, lpq, rings.runs, load_const(rings.kern + k), rq);
+                            if (need) prefix_pairs<GP>(lane_q, rings.pairs, load_const(rings.pair_kern + k), rq);
                             const float wq = as_const(rings.scale_q)[k];  // w_s 2^-F, as the forward
 #pragma unroll
                             for (int p = 0; p < GP; ++p)

@@ -39,6 +39,13 @@ struct DevRings {
     const int4 *runs;
     const int4 *flip;  // the ring runs rotated by 180 degrees (same kern ranges)
     const int2 *kern;
+    // The ring runs for a fixed-point prefix table of one row pitch lp, as
+    // byte offsets from a lane's first prefix value: pairs[i] = {a0, b0, a1,
+    // b1} for two consecutive runs of a kernel, a = 4 (row lp + col) and
+    // b = a + 4 len (a run adds Q[b] - Q[a]); pair_kern[k] = {first pair,
+    // number of runs} (an odd kernel's last pair uses only a0, b0).
+    const int4 *pairs;
+    const int2 *pair_kern;
     const float *scale;    // w_s per kernel
     const float *scale_q;  // w_s * 2^-F per kernel (exact: a power-of-two scaling)
     float qscale;  // 2^F: fixed-point scale of tile values for ring sums
@@ -102,6 +109,9 @@ __host__ __device__ constexpr int lds_floats(int tw, int th, int r, int pm = 1)
 {
     return (th + 2 * r) * lds_pitch(tw, r, pm);
 }
+// Row pitch of the fixed-point prefix table of a tw-wide tile (one leading
+// column, Q[r][0] = 0).
+__host__ __device__ constexpr int prefix_pitch(int tw, int r, int pm = 1) { return lds_pitch(tw + 1, r, pm); }
 
 // One image of an adjoint batch: the forward's image (x, y = the forward's
 // output, map = its scale indices, tab) plus the upstream gradient g, the

@@ -24,11 +24,11 @@
 namespace lpa {
 namespace {
 
+typedef float float4_words __attribute__((ext_vector_type(4)));
+
 // The P values of a lane at consecutive addresses: as P / 4 words of four
 // elements when `whole` (every pixel inside, the row start aligned to four
 // elements), else element by element for the pixels inside the image.
-typedef float float4_words __attribute__((ext_vector_type(4)));
-
 template <int P, typename T, typename G>
 __device__ __forceinline__ void store_row(G *dst, const T (&v)[P], const bool (&inside)[P], bool whole)
 {
@@ -111,7 +111,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
         if constexpr (PEEL) {
             // yhat_0 = (w_0 2^-F) * (fixed-point sum over the support of
             // scale 0); a lane with no pixel inside the image skips the reads
-            if (any_inside) prefix_runs<P>(src, lp, rings.runs, load_const(rings.kern + d * S), ring_q);
+            if (any_inside) prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + d * S), ring_q);
             const float wq = as_const(rings.scale_q)[d * S];
             const float thr = as_const(thr_t)[d * S];
 #pragma unroll
@@ -146,7 +146,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 // yhat_s = (w_s 2^-F) * (fixed-point sum over the support of
                 // scale s), the previous scale's sum plus this scale's ring;
                 // one rounding for the product, none hidden in the interval.
-                if (alive) prefix_runs<P>(src, lp, rings.runs, load_const(rings.kern + k), ring_q);
+                if (alive) prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + k), ring_q);
                 const float wq = as_const(rings.scale_q)[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
@@ -231,7 +231,7 @@ __device__ __forceinline__ void process_tile(float *__restrict__ tile, const Dev
                                              float *__restrict__ y, uint8_t *__restrict__ map,
                                              float *__restrict__ est, int W, int H, int pitch, int x0, int y0, int R)
 {
-    const int lp = lds_pitch(TW + 1, R, PM);
+    const int lp = prefix_pitch(TW, R, PM);  // the ring pairs' offsets are in this pitch
     const int tid = threadIdx.x;
     int tx, ty;
     lane_cell<TW, TH, P, NT, SPLIT>(tid, tx, ty);
@@ -309,7 +309,7 @@ __device__ __forceinline__ void estimate_lane(const T *__restrict__ src, int lp,
             const int k = d * S + s;
             float acc[P];
             if constexpr (MODE == kWalkRing) {
-                if (need) prefix_runs<P>(src, lp, rings.runs, load_const(rings.kern + k), ring_q);
+                if (need) prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + k), ring_q);
                 const float wq = as_const(rings.scale_q)[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(kShapes[SH].nt) void estimate_kernel(DevKset ks, De
     extern __shared__ float tile[];
     __shared__ int next;
     constexpr TileShape sh = kShapes[SH];
-    const int lp = lds_pitch(sh.tw + 1, R, sh.pm);
+    const int lp = prefix_pitch(sh.tw, R, sh.pm);  // the ring pairs' offsets are in this pitch
     const int tid = threadIdx.x;
     int tx, ty;
     lane_cell<sh.tw, sh.th, sh.px, sh.nt, sh.split>(tid, tx, ty);

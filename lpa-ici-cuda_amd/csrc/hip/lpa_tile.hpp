@@ -428,6 +428,48 @@ __device__ __forceinline__ void prefix_runs(const uint32_t *__restrict__ lane, i
     }
 }
 
+// prefix_runs with the runs as byte-offset pairs (DevRings::pairs) of the
+// table's row pitch: one scalar load brings both runs' four boundaries, and
+// no scalar arithmetic forms them.  kp = {first pair, number of runs}.
+template <int P>
+__device__ __forceinline__ void prefix_pairs(const uint32_t *__restrict__ lane, const int4 *__restrict__ pairs,
+                                             int2 kp, uint32_t (&rs)[P])
+{
+    const char *__restrict__ base = reinterpret_cast<const char *>(lane);
+    const int4 *__restrict__ pp = pairs + kp.x;
+    int r = 0;
+    for (; r + 1 < kp.y; r += 2, ++pp) {
+        const int4 o = load_const(pp);
+        const uint32_t *__restrict__ ua = reinterpret_cast<const uint32_t *>(base + o.x);
+        const uint32_t *__restrict__ ub = reinterpret_cast<const uint32_t *>(base + o.y);
+        const uint32_t *__restrict__ va = reinterpret_cast<const uint32_t *>(base + o.z);
+        const uint32_t *__restrict__ vb = reinterpret_cast<const uint32_t *>(base + o.w);
+        uint32_t a0[P], b0[P], a1[P], b1[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            a0[p] = ua[p];
+            b0[p] = ub[p];
+            a1[p] = va[p];
+            b1[p] = vb[p];
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) rs[p] = add3(rs[p], b0[p], b1[p]) - a0[p] - a1[p];
+    }
+    if (r < kp.y) {
+        const int2 o = load_const(reinterpret_cast<const int2 *>(pp));
+        const uint32_t *__restrict__ ua = reinterpret_cast<const uint32_t *>(base + o.x);
+        const uint32_t *__restrict__ ub = reinterpret_cast<const uint32_t *>(base + o.y);
+        uint32_t a0[P], b0[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            a0[p] = ua[p];
+            b0[p] = ub[p];
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) rs[p] += b0[p] - a0[p];
+    }
+}
+
 // The same fixed-point sums from a float tile (quantizing each value read):
 // rq[p] += sum over the runs kr of q(tile[row, col + p + j]).
 template <int L, int P>
