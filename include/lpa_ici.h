@@ -172,7 +172,7 @@ void lpa_plan_destroy(lpa_plan_t plan);
 typedef struct lpa_tile_config {
     int tile_w, tile_h;   /* output pixels per work-group tile */
     int px_per_lane;      /* adjacent output pixels one lane computes */
-    int body;             /* inner loop: 1 = one unrolled body per run length, 0 = sliding-window loop */
+    int body;             /* inner loop: 2 = one chain of unrolled taps, left after a run's last, 0 = sliding-window loop */
     int halo;             /* LDS halo radius, >= the kernel set's max radius */
     int blocks_per_cu;    /* persistent work-groups per CU (batched form) at the full halo; 0 without a device */
 } lpa_tile_config;

@@ -40,10 +40,6 @@ struct lpa_kset {
     float *d_weights = nullptr;
     int2 *d_kern = nullptr;
     // Stretches of consecutive runs of one length (see lpa::DevKset).
-    std::vector<int4> groups;
-    std::vector<int2> kern_groups;
-    int4 *d_groups = nullptr;
-    int2 *d_kern_groups = nullptr;
     int shape = -1;  // -1: built-in choice
     // Adjoint tables (see lpa::DevAdjoint).
     std::vector<int4> runs_flip;
@@ -75,7 +71,7 @@ struct lpa_kset {
     std::mutex upload_mutex;
     lpa::DevKset dev() const
     {
-        return lpa::DevKset{d_runs, d_weights, d_kern, d_groups, d_kern_groups, D, S};
+        return lpa::DevKset{d_runs, d_weights, d_kern, D, S};
     }
     // The ring runs as byte-offset pairs, one table per prefix row pitch that
     // a tile configuration or the adjoint uses (all built at upload).
@@ -103,7 +99,7 @@ struct lpa_kset {
     }
     lpa::DevAdjoint adjoint() const
     {
-        return lpa::DevAdjoint{lpa::DevKset{d_runs_flip, d_weights_flip, d_kern, nullptr, nullptr, D, S},
+        return lpa::DevAdjoint{lpa::DevKset{d_runs_flip, d_weights_flip, d_kern, D, S},
                                d_tap_lds,
                                d_tap_kernel,
                                d_kern_w,
@@ -186,17 +182,6 @@ void pack_runs(lpa_kset *ks, const float *dense, const int *offsets, const int *
         ks->num_nonzero += nz;
     }
     ks->radius = radius;
-    // Each kernel's stretches of consecutive runs of one length: a walk
-    // dispatches on the length once per stretch, in the runs' own order.
-    for (int k = 0; k < ks->K; ++k) {
-        const int2 kr = ks->kern[k];
-        const int first_group = (int)ks->groups.size();
-        for (int r = kr.x; r < kr.x + kr.y; ++r) {
-            if (r == kr.x || ks->runs[r].z != ks->runs[r - 1].z) ks->groups.push_back(int4{r, 0, ks->runs[r].z, 0});
-            ks->groups.back().y += 1;
-        }
-        ks->kern_groups.push_back(int2{first_group, (int)ks->groups.size() - first_group});
-    }
     // Adjoint tables: tap positions and owners, and the rotated runs.
     ks->weights_flip.resize(ks->weights.size());
     ks->tap_lds.resize(ks->weights.size());
@@ -343,31 +328,24 @@ lpa_status ensure_uploaded(lpa_kset *ks)
     hipError_t e = hipGetDevice(&ks->device);
     const size_t nr = std::max<size_t>(1, ks->runs.size()), nw = std::max<size_t>(1, ks->weights.size());
     if (e == hipSuccess) e = hipMalloc(&ks->d_runs, sizeof(int4) * nr);
-    // Weights get 4 floats of slack: the 4-tap loop never reads past a run,
-    // but the slack keeps any scalar prefetch of the last run in bounds.
-    if (e == hipSuccess) e = hipMalloc(&ks->d_weights, sizeof(float) * (nw + 4));
+    // Weights get kMaxRun floats of slack: the chain body loads kMaxRun
+    // weights from a run's first, whatever its length.
+    if (e == hipSuccess) e = hipMalloc(&ks->d_weights, sizeof(float) * (nw + lpa::kMaxRun));
     if (e == hipSuccess) e = hipMalloc(&ks->d_kern, sizeof(int2) * K);
-    if (e == hipSuccess) e = hipMemset(ks->d_weights, 0, sizeof(float) * (nw + 4));
+    if (e == hipSuccess) e = hipMemset(ks->d_weights, 0, sizeof(float) * (nw + lpa::kMaxRun));
     if (e == hipSuccess && !ks->runs.empty())
         e = hipMemcpy(ks->d_runs, ks->runs.data(), sizeof(int4) * ks->runs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !ks->weights.empty())
         e = hipMemcpy(ks->d_weights, ks->weights.data(), sizeof(float) * ks->weights.size(),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ks->d_kern, ks->kern.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
-    const size_t ng = std::max<size_t>(1, ks->groups.size());
-    if (e == hipSuccess) e = hipMalloc(&ks->d_groups, sizeof(int4) * ng);
-    if (e == hipSuccess) e = hipMalloc(&ks->d_kern_groups, sizeof(int2) * K);
-    if (e == hipSuccess && !ks->groups.empty())
-        e = hipMemcpy(ks->d_groups, ks->groups.data(), sizeof(int4) * ks->groups.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(ks->d_kern_groups, ks->kern_groups.data(), sizeof(int2) * K, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&ks->d_runs_flip, sizeof(int4) * nr);
-    if (e == hipSuccess) e = hipMalloc(&ks->d_weights_flip, sizeof(float) * (nw + 4));
+    if (e == hipSuccess) e = hipMalloc(&ks->d_weights_flip, sizeof(float) * (nw + lpa::kMaxRun));
     if (e == hipSuccess) e = hipMalloc(&ks->d_tap_lds, sizeof(int) * nw);
     if (e == hipSuccess) e = hipMalloc(&ks->d_tap_kernel, sizeof(int) * nw);
     if (e == hipSuccess) e = hipMalloc(&ks->d_kern_w, sizeof(int4) * K);
     if (e == hipSuccess) e = hipMalloc(&ks->d_sumsq, sizeof(float) * K);
-    if (e == hipSuccess) e = hipMemset(ks->d_weights_flip, 0, sizeof(float) * (nw + 4));
+    if (e == hipSuccess) e = hipMemset(ks->d_weights_flip, 0, sizeof(float) * (nw + lpa::kMaxRun));
     if (e == hipSuccess && !ks->runs.empty())
         e = hipMemcpy(ks->d_runs_flip, ks->runs_flip.data(), sizeof(int4) * ks->runs.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !ks->weights.empty()) {
@@ -620,7 +598,7 @@ void lpa_kset_destroy(lpa_kset_t ks)
     if (ks->d_runs) (void)hipFree(ks->d_runs);
     if (ks->d_weights) (void)hipFree(ks->d_weights);
     if (ks->d_kern) (void)hipFree(ks->d_kern);
-    for (void *p : {(void *)ks->d_groups, (void *)ks->d_kern_groups, (void *)ks->d_runs_flip, (void *)ks->d_weights_flip, (void *)ks->d_tap_lds,
+    for (void *p : {(void *)ks->d_runs_flip, (void *)ks->d_weights_flip, (void *)ks->d_tap_lds,
                     (void *)ks->d_tap_kernel, (void *)ks->d_kern_w, (void *)ks->d_sumsq, (void *)ks->d_ring_runs,
                     (void *)ks->d_ring_runs_flip, (void *)ks->d_ring_kern, (void *)ks->d_ring_scale})
         if (p) (void)hipFree(p);

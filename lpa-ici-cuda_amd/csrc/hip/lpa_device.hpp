@@ -24,8 +24,6 @@ struct DevKset {
     const int4 *runs;
     const float *weights;
     const int2 *kern;
-    const int4 *groups;  // null for tables walked run by run only
-    const int2 *kern_groups;
     int D, S;
 };
 
@@ -69,8 +67,8 @@ struct ImgDesc {
 };
 
 // Compiled tile shapes: output tile tw x th pixels, px adjacent pixels per
-// lane along a row, nt lanes per work-group, the inner-loop body (1:
-// length-dispatched unrolled runs, 0: sliding-window loop), the LDS row
+// lane along a row, nt lanes per work-group, the inner-loop body (2: the
+// chain of unrolled taps with exits, 0: sliding-window loop), the LDS row
 // pitch's residue mod 32 (the bank offset between consecutive rows), the
 // lane layout (split 1: each half-wave, whose 32 lanes the LDS serves
 // together, takes its own half of the rows' lanes, so the 32 lanes of a read
@@ -82,17 +80,18 @@ struct TileShape {
 constexpr int kThreads = 256;
 constexpr int kNumShapes = 10;
 constexpr TileShape kShapes[kNumShapes] = {
-    {32, 32, 4, 1, 1, 256, 0, 1},   // 0: wave covers 32x8 px
+    {32, 32, 4, 2, 1, 256, 0, 1},   // 0: wave covers 32x8 px
     {32, 32, 4, 0, 1, 256, 0, 1},   // 1: same, window loop
-    {64, 16, 4, 1, 1, 256, 0, 1},   // 2: wave covers 64x4 px
-    {32, 16, 2, 1, 1, 256, 0, 1},   // 3: wave covers 32x4 px
-    {64, 32, 8, 1, 1, 256, 0, 1},   // 4: wave covers 64x8 px
-    {32, 64, 8, 1, 1, 256, 0, 1},   // 5: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
-    {32, 64, 8, 1, 2, 256, 0, 1},   // 6: as 5, rows two banks apart: the dword pairs of 16 lanes (4 rows) are disjoint
-    {64, 64, 8, 1, 1, 512, 1, 4},   // 7: halo 3.9 words per output pixel against 5.8; two work-groups per CU
-    {64, 32, 4, 1, 1, 512, 1, 6},   // 8: three work-groups of 512 lanes per CU (24 waves)
-    {64, 64, 4, 1, 1, 1024, 1, 8},  // 9: 1024 lanes within 64 VGPRs: two work-groups per CU (32 waves)
+    {64, 16, 4, 2, 1, 256, 0, 1},   // 2: wave covers 64x4 px
+    {32, 16, 2, 2, 1, 256, 0, 1},   // 3: wave covers 32x4 px
+    {64, 32, 8, 2, 1, 256, 0, 1},   // 4: wave covers 64x8 px
+    {32, 64, 8, 2, 1, 256, 0, 1},   // 5: wave covers 32x16 px; 4 lanes per row keep 8-px lanes in distinct banks
+    {32, 64, 8, 2, 2, 256, 0, 1},   // 6: as 5, rows two banks apart: the dword pairs of 16 lanes (4 rows) are disjoint
+    {64, 64, 8, 2, 1, 512, 1, 4},   // 7: halo 3.9 words per output pixel against 5.8; two work-groups per CU
+    {64, 32, 4, 2, 1, 512, 1, 6},   // 8: three work-groups of 512 lanes per CU (24 waves)
+    {64, 64, 4, 2, 1, 1024, 1, 8},  // 9: 1024 lanes within 64 VGPRs: two work-groups per CU (32 waves)
 };
+
 constexpr int kMaxRadius = 31;
 // LDS of one work-group on gfx950.
 constexpr int kMaxLdsBytes = 160 * 1024;

@@ -110,8 +110,8 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
         constexpr int PEEL = MODE == kWalkRing ? 1 : 0;
         if constexpr (PEEL) {
             // yhat_0 = (w_0 2^-F) * (fixed-point sum over the support of
-            // scale 0); a lane with no pixel inside the image skips the reads
-            if (any_inside) prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + d * S), ring_q);
+            // scale 0)
+            prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + d * S), ring_q);
             const float wq = as_const(rings.scale_q)[d * S];
             const float thr = as_const(thr_t)[d * S];
 #pragma unroll
@@ -131,39 +131,41 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 sel[p] = 0;
             }
         }
-        // alive: some pixel of the lane inside the image still has a
-        // non-empty interval (it is the `ok` of the previous scale; an empty
+        // live: the wave's lanes with a pixel inside the image whose interval
+        // is still non-empty (the `ok` of the previous scale; an empty
         // intersection stays empty).  The wave stops walking this direction
-        // once no lane is.  Pixels outside the image are walked along with
-        // the others but keep no lane alive, and nothing of theirs is stored.
-        bool alive = any_inside;
-        uint64_t live = __builtin_amdgcn_ballot_w64(alive);  // the wave's alive lanes
+        // once no lane is live.  Every lane of a walking wave walks, as a
+        // lane masked off would cost the same issue cycles: a pixel whose
+        // interval is empty keeps it empty and its selection, and pixels
+        // outside the image keep no lane live and store nothing.
+        uint64_t live = __builtin_amdgcn_ballot_w64(any_inside);
         for (int s = PEEL; s < S; ++s) {
             if (live == 0) break;
             const int k = d * S + s;
+            // the scale's table entries are read before the lane-divergent
+            // walk: read after it, their index is no longer known to be
+            // wave-uniform and the reads leave the scalar unit
+            const float thr = as_const(thr_t)[k];
             float acc[P];
             if constexpr (MODE == kWalkRing) {
                 // yhat_s = (w_s 2^-F) * (fixed-point sum over the support of
                 // scale s), the previous scale's sum plus this scale's ring;
                 // one rounding for the product, none hidden in the interval.
-                if (alive) prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + k), ring_q);
                 const float wq = as_const(rings.scale_q)[k];
+                prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + k), ring_q);
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {
                 // yhat_s = w_s * (float sum over the support of scale s).
-                if (alive)
-                    correlate_runs<P, false, 1, false>(src, lp, rings.runs, nullptr, load_const(rings.kern + k),
-                                                       ring_sum);
                 const float w = as_const(rings.scale)[k];
+                ring_sums<P>(src, lp, rings.runs, load_const(rings.kern + k), ring_sum);
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;
             } else {
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = 0.0f;
-                if (alive) correlate<P, EXACT, BODY>(src, lp, ks, k, acc);
+                correlate<P, EXACT, BODY>(src, lp, ks, k, acc);
             }
-            const float thr = as_const(thr_t)[k];
             bool any_ok = false;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
@@ -178,7 +180,6 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 }
                 any_ok |= ok && inside[p];
             }
-            alive = any_ok;
             live = __builtin_amdgcn_ballot_w64(any_ok);
         }
         // 1/var of the selected scale, read once per direction (the
@@ -315,8 +316,7 @@ __device__ __forceinline__ void estimate_lane(const T *__restrict__ src, int lp,
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {
                 if (need)
-                    correlate_runs<P, false, 1, false>(src, lp, rings.runs, nullptr, load_const(rings.kern + k),
-                                                       ring_sum);
+                    ring_sums<P>(src, lp, rings.runs, load_const(rings.kern + k), ring_sum);
                 const float w = as_const(rings.scale)[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;
@@ -550,7 +550,8 @@ hipError_t estimate_launch(const DevKset &ks, const DevRings &rings, const EstDe
                            int num_tiles, int *d_counter, int radius, int grid, hipStream_t stream)
 {
     constexpr TileShape sh = kShapes[SH];
-    const size_t lds = sizeof(float) * (size_t)lds_floats(sh.tw + 1, sh.th, radius, sh.pm);
+    // two words past the window: the chain body reads one value past a run
+    const size_t lds = sizeof(float) * ((size_t)lds_floats(sh.tw + 1, sh.th, radius, sh.pm) + 2);
     const hipError_t e = allow_lds(estimate_kernel<SH, MODE>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((estimate_kernel<SH, MODE>), dim3(grid), dim3(sh.nt), lds, stream, ks, rings, d_imgs, d_tiles,

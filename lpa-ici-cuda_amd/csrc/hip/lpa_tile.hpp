@@ -1,5 +1,5 @@
 // Device helpers shared by the forward and adjoint kernels: the tap update,
-// the sliding-window run correlation and the LDS tile staging.
+// the run correlations and the LDS tile staging.
 #ifndef LPA_TILE_HPP
 #define LPA_TILE_HPP
 
@@ -90,32 +90,6 @@ __device__ __forceinline__ float tap(float acc, float w, float v)
         return __builtin_fmaf(w, v, acc);
 }
 
-// One run of exactly L taps for the P pixels starting at `src`: L + P - 1
-// tile values and L weights, then L*P FMAs.  Per pixel the taps are added in
-// order along the run.
-template <int L, int P, bool EXACT, bool WEIGHTED = true>
-__device__ __forceinline__ void run_taps(const float *__restrict__ src, const float *__restrict__ w, float (&acc)[P])
-{
-    float v[L + P - 1];
-#pragma unroll
-    for (int q = 0; q < L + P - 1; ++q) v[q] = src[q];
-    if constexpr (WEIGHTED) {
-        float c[L];
-#pragma unroll
-        for (int j = 0; j < L; ++j) c[j] = as_const(w)[j];
-#pragma unroll
-        for (int j = 0; j < L; ++j)
-#pragma unroll
-            for (int p = 0; p < P; ++p) acc[p] = tap<EXACT>(acc[p], c[j], v[j + p]);
-    } else {
-        (void)w;
-#pragma unroll
-        for (int j = 0; j < L; ++j)
-#pragma unroll
-            for (int p = 0; p < P; ++p) acc[p] = acc[p] + v[j + p];
-    }
-}
-
 // Sliding-window body: a window of P+3 tile values moves along the run four
 // taps at a time, then one tap at a time for the rest.  Smaller code than the
 // unrolled body; costs window moves and a loop tail.
@@ -156,87 +130,105 @@ __device__ __forceinline__ void run_window(const float *__restrict__ src, const 
 constexpr int kMaxRun = 32;
 
 // acc[p] += sum over the runs kr = {first, count} of w[j] * tile[row, col + p + j]
-// (or of tile[row, col + p + j] alone when !WEIGHTED), for the P pixels
-// starting at `lane` (the LDS address of the lane's first pixel).  Per pixel
-// the taps are added in run order, then in order along the run: the
-// row-major order of the dense kernel.  Each LDS value read feeds up to P
-// FMAs; the run's address costs one add from the lane's base; weights and
-// run records are wave-uniform scalar loads.  BODY 1: the run length is the
-// same for every lane, so a switch picks one fully unrolled body per run (no
-// window shuffling, no loop tail).  BODY 0: the sliding-window loop.
-template <int P, bool EXACT, int BODY, bool WEIGHTED>
-__device__ __forceinline__ void correlate_runs(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
-                                               const float *__restrict__ weights, int2 kr, float (&acc)[P])
+// for the P pixels starting at `lane` (the LDS address of the lane's first
+// pixel), with the sliding-window body (BODY 0).  Per pixel the taps are
+// added in run order, then in order along the run: the row-major order of
+// the dense kernel.  Weights and run records are wave-uniform scalar loads.
+template <int P, bool EXACT>
+__device__ __forceinline__ void correlate_window(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
+                                                 const float *__restrict__ weights, int2 kr, float (&acc)[P])
 {
     for (int r = 0; r < kr.y; ++r) {
         const int4 run = load_const(runs + kr.x + r);
         const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
-        const float *__restrict__ src = lane + off;
-        const float *__restrict__ w = weights + run.w;
-        if constexpr (BODY == 0 && WEIGHTED) {
-            run_window<P, EXACT>(src, w, run.z, acc);
-            continue;
-        }
-        switch (run.z) {
-#define LPA_RUN_CASE(L)                                                                                            \
-    case L: run_taps<L, P, EXACT, WEIGHTED>(src, w, acc); break;
-            LPA_RUN_CASE(1) LPA_RUN_CASE(2) LPA_RUN_CASE(3) LPA_RUN_CASE(4) LPA_RUN_CASE(5) LPA_RUN_CASE(6)
-            LPA_RUN_CASE(7) LPA_RUN_CASE(8) LPA_RUN_CASE(9) LPA_RUN_CASE(10) LPA_RUN_CASE(11) LPA_RUN_CASE(12)
-            LPA_RUN_CASE(13) LPA_RUN_CASE(14) LPA_RUN_CASE(15) LPA_RUN_CASE(16) LPA_RUN_CASE(17) LPA_RUN_CASE(18)
-            LPA_RUN_CASE(19) LPA_RUN_CASE(20) LPA_RUN_CASE(21) LPA_RUN_CASE(22) LPA_RUN_CASE(23) LPA_RUN_CASE(24)
-            LPA_RUN_CASE(25) LPA_RUN_CASE(26) LPA_RUN_CASE(27) LPA_RUN_CASE(28) LPA_RUN_CASE(29) LPA_RUN_CASE(30)
-            LPA_RUN_CASE(31) LPA_RUN_CASE(32)
-#undef LPA_RUN_CASE
-        default: break;
-        }
+        run_window<P, EXACT>(lane + off, weights + run.w, run.z, acc);
     }
 }
 
-// correlate_runs over a kernel's stretches of consecutive runs of one length
-// ({first run, number of runs, length}): one dispatch on the length per
-// stretch, then a loop over its runs with the unrolled body.  The taps are
-// added in the same order as run by run.  Each dispatch leaves the sums in
-// the registers the switch's cases share and copies them back to the loop's,
-// so the fewer dispatches, the fewer copies and scalar branches.
-template <int P, bool EXACT>
-__device__ __forceinline__ void correlate_groups(const float *__restrict__ lane, int lp,
-                                                 const int4 *__restrict__ runs, const float *__restrict__ weights,
-                                                 const int4 *__restrict__ groups, int2 kg, float (&acc)[P])
+// The chain body's steps, nested so that each one is entered only when its
+// predecessor did not end the run: reads I and I + 1 of the window, then the
+// next pair while values remain; tap J for the P pixels, then the next tap
+// while taps remain.
+template <int I, int NV>
+__device__ __forceinline__ void chain_reads(const float *__restrict__ src, int nv, float (&v)[NV])
 {
-    for (int g = 0; g < kg.y; ++g) {
-        const int4 grp = load_const(groups + kg.x + g);
-        switch (grp.z) {
-#define LPA_GROUP_CASE(L)                                                                                          \
-    case L:                                                                                                        \
-        for (int r = 0; r < grp.y; ++r) {                                                                          \
-            const int4 run = load_const(runs + grp.x + r);                                                         \
-            const float *__restrict__ src = lane + __builtin_amdgcn_readfirstlane(run.x * lp + run.y);             \
-            run_taps<L, P, EXACT, true>(src, weights + run.w, acc);                                                \
-        }                                                                                                          \
-        break;
-            LPA_GROUP_CASE(1) LPA_GROUP_CASE(2) LPA_GROUP_CASE(3) LPA_GROUP_CASE(4) LPA_GROUP_CASE(5)
-            LPA_GROUP_CASE(6) LPA_GROUP_CASE(7) LPA_GROUP_CASE(8) LPA_GROUP_CASE(9) LPA_GROUP_CASE(10)
-            LPA_GROUP_CASE(11) LPA_GROUP_CASE(12) LPA_GROUP_CASE(13) LPA_GROUP_CASE(14) LPA_GROUP_CASE(15)
-            LPA_GROUP_CASE(16) LPA_GROUP_CASE(17) LPA_GROUP_CASE(18) LPA_GROUP_CASE(19) LPA_GROUP_CASE(20)
-            LPA_GROUP_CASE(21) LPA_GROUP_CASE(22) LPA_GROUP_CASE(23) LPA_GROUP_CASE(24) LPA_GROUP_CASE(25)
-            LPA_GROUP_CASE(26) LPA_GROUP_CASE(27) LPA_GROUP_CASE(28) LPA_GROUP_CASE(29) LPA_GROUP_CASE(30)
-            LPA_GROUP_CASE(31) LPA_GROUP_CASE(32)
-#undef LPA_GROUP_CASE
-        default: break;
+    v[I] = src[I];
+    if constexpr (I + 1 < NV) v[I + 1] = src[I + 1];
+    if constexpr (I + 2 < NV)
+        if (I + 2 < nv) chain_reads<I + 2, NV>(src, nv, v);
+}
+template <int J, int P, bool EXACT, bool WEIGHTED, int NV>
+__device__ __forceinline__ void chain_taps(const float (&v)[NV], const float (&c)[kMaxRun], int len, float (&acc)[P])
+{
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if constexpr (WEIGHTED)
+            acc[p] = tap<EXACT>(acc[p], c[J], v[J + p]);
+        else
+            acc[p] = acc[p] + v[J + p];
+    }
+    if constexpr (J + 1 < kMaxRun)
+        if (J + 1 < len) chain_taps<J + 1, P, EXACT, WEIGHTED, NV>(v, c, len, acc);
+}
+
+// The run correlation with no dispatch on the run length (BODY 2).  Per run, the
+// L + P - 1 tile values are read two at a time into one register window,
+// leaving the reads once they are all in; the kMaxRun weights are one scalar
+// load (the weight table is padded by kMaxRun words); then the taps go in
+// order, leaving after tap L.  Every length runs the same code on the same
+// registers, so the sums are never copied between the bodies of different
+// lengths, as the cases of a switch on the length would.  Per pixel the taps
+// are added in run order, then in order along the run: the row-major order
+// of the dense kernel.  An odd L + P - 1 reads one value past
+// the run's window: the next word of the same LDS allocation, never used.
+typedef float Words16 __attribute__((ext_vector_type(16), aligned(4)));
+static_assert(kMaxRun == 32, "the chain body loads its weights as two 16-word vectors");
+
+template <int P, bool EXACT, bool WEIGHTED>
+__device__ __forceinline__ void correlate_chain(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
+                                                const float *__restrict__ weights, int2 kr, float (&acc)[P])
+{
+    constexpr int NV = kMaxRun + P - 1;
+    for (int r = 0; r < kr.y; ++r) {
+        const int4 run = load_const(runs + kr.x + r);
+        const float *__restrict__ src = lane + __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
+        float c[kMaxRun];
+        if constexpr (WEIGHTED) {
+            // two 16-word loads: a load per tap would be sunk into the tap
+            // chain, each with its own wait
+            const auto cw = reinterpret_cast<const __attribute__((address_space(4))) Words16 *>(
+                as_const(weights + run.w));
+            const Words16 lo = cw[0], hi = cw[1];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                c[j] = lo[j];
+                c[16 + j] = hi[j];
+            }
         }
+        float v[NV];
+        chain_reads<0, NV>(src, run.z + P - 1, v);
+        chain_taps<0, P, EXACT, WEIGHTED, NV>(v, c, run.z, acc);
     }
 }
 
-// Weighted correlation of kernel k of a set (see correlate_runs): the
-// unrolled body goes by stretches of runs of one length.
-template <int P, bool EXACT, int BODY = 1>
+// Weighted correlation of kernel k of a set: BODY 2, the chain body; BODY 0,
+// the sliding window.
+template <int P, bool EXACT, int BODY = 2>
 __device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp, const DevKset &ks, int k,
                                           float (&acc)[P])
 {
-    if constexpr (BODY == 1)
-        correlate_groups<P, EXACT>(lane, lp, ks.runs, ks.weights, ks.groups, load_const(ks.kern_groups + k), acc);
+    if constexpr (BODY == 2)
+        correlate_chain<P, EXACT, true>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
     else
-        correlate_runs<P, EXACT, BODY, true>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
+        correlate_window<P, EXACT>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
+}
+
+// Unweighted sums over the runs kr (the float ring walk's ring sums).
+template <int P>
+__device__ __forceinline__ void ring_sums(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs, int2 kr,
+                                          float (&acc)[P])
+{
+    correlate_chain<P, false, false>(lane, lp, runs, nullptr, kr, acc);
 }
 
 // Stage the (TW + 2R) x (TH + 2R) window whose top-left image pixel is
@@ -467,43 +459,6 @@ __device__ __forceinline__ void prefix_pairs(const uint32_t *__restrict__ lane, 
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) rs[p] += b0[p] - a0[p];
-    }
-}
-
-// The same fixed-point sums from a float tile (quantizing each value read):
-// rq[p] += sum over the runs kr of q(tile[row, col + p + j]).
-template <int L, int P>
-__device__ __forceinline__ void run_taps_q(const float *__restrict__ src, float qscale, uint32_t (&rq)[P])
-{
-    uint32_t v[L + P - 1];
-#pragma unroll
-    for (int i = 0; i < L + P - 1; ++i) v[i] = quantize(src[i], qscale);
-#pragma unroll
-    for (int j = 0; j < L; ++j)
-#pragma unroll
-        for (int p = 0; p < P; ++p) rq[p] += v[j + p];
-}
-template <int P>
-__device__ __forceinline__ void quantized_runs(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
-                                               int2 kr, float qscale, uint32_t (&rq)[P])
-{
-    for (int r = 0; r < kr.y; ++r) {
-        const int4 run = load_const(runs + kr.x + r);
-        const int off = __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
-        const float *__restrict__ src = lane + off;
-        switch (run.z) {
-#define LPA_QRUN_CASE(L)                                                                                           \
-    case L: run_taps_q<L, P>(src, qscale, rq); break;
-            LPA_QRUN_CASE(1) LPA_QRUN_CASE(2) LPA_QRUN_CASE(3) LPA_QRUN_CASE(4) LPA_QRUN_CASE(5) LPA_QRUN_CASE(6)
-            LPA_QRUN_CASE(7) LPA_QRUN_CASE(8) LPA_QRUN_CASE(9) LPA_QRUN_CASE(10) LPA_QRUN_CASE(11)
-            LPA_QRUN_CASE(12) LPA_QRUN_CASE(13) LPA_QRUN_CASE(14) LPA_QRUN_CASE(15) LPA_QRUN_CASE(16)
-            LPA_QRUN_CASE(17) LPA_QRUN_CASE(18) LPA_QRUN_CASE(19) LPA_QRUN_CASE(20) LPA_QRUN_CASE(21)
-            LPA_QRUN_CASE(22) LPA_QRUN_CASE(23) LPA_QRUN_CASE(24) LPA_QRUN_CASE(25) LPA_QRUN_CASE(26)
-            LPA_QRUN_CASE(27) LPA_QRUN_CASE(28) LPA_QRUN_CASE(29) LPA_QRUN_CASE(30) LPA_QRUN_CASE(31)
-            LPA_QRUN_CASE(32)
-#undef LPA_QRUN_CASE
-        default: break;
-        }
     }
 }
 
