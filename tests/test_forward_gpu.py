@@ -217,15 +217,32 @@ def test_fixed_point_fallback_windows(ks):
 def test_tile_shapes_identical(ks, shape):
     """Every tile shape adds each pixel's taps in the same order: same bits."""
     x = noisy_synth(200, 136, seed=5)
-    base = run(ks, x, flags=lpa_ici.FWD_EXACT, maps=False)
+    walks = [lpa_ici.FWD_EXACT, 0, lpa_ici.FWD_WEIGHTED, lpa_ici.FWD_RING_FLOAT]
+    base = [run(ks, x, flags=f) for f in walks]
     ks.set_tile_config(shape)
     try:
-        y = run(ks, x, flags=lpa_ici.FWD_EXACT, maps=False)
-        yf = run(ks, x, maps=False)
+        got = [run(ks, x, flags=f) for f in walks]
     finally:
         ks.set_tile_config(-1)
-    assert np.array_equal(y, base)
-    assert np.array_equal(yf, run(ks, x, maps=False))
+    for f, (yb, mb), (y, m) in zip(walks, base, got):
+        assert np.array_equal(y.view(np.uint32), yb.view(np.uint32)), f
+        assert np.array_equal(m, mb), f
+
+
+@pytest.mark.parametrize("shape", range(len(lpa_ici.tile_configs())))
+def test_tile_shapes_identical_weighted_set(shape):
+    """The same for a set with no ring form (the m=(2,0) gaussian set): the
+    weighted walk, exact and fast, gives the same bits on every tile shape."""
+    ksg = lpa_ici.load_kernels(lpa_ici.GAUSSIAN_M20_KERNELS)
+    x = noisy_synth(136, 200, seed=6)
+    walks = [lpa_ici.FWD_EXACT, 0]
+    base = [run(ksg, x, flags=f) for f in walks]
+    ksg.set_tile_config(shape)
+    got = [run(ksg, x, flags=f) for f in walks]
+    for f, (yb, mb), (y, m) in zip(walks, base, got):
+        assert np.array_equal(y.view(np.uint32), yb.view(np.uint32)), f
+        assert np.array_equal(m, mb), f
+    ksg.close()
 
 
 @pytest.mark.parametrize("w,h", [(1, 1), (1, 9), (9, 1), (3, 3), (33, 65), (100, 3), (31, 31), (129, 47)])
