@@ -304,13 +304,15 @@ def fwd_bwd_leg(ks, steps, warmup, stream):
     dys = [torch.randn(x.shape, generator=g, device="cuda") for x in xs]
     dxs = [torch.empty_like(x) for x in xs]
     dk = torch.zeros(ks.info()["num_weights"], dtype=torch.float32, device="cuda")
+    # the persistent forward at its own tile configuration for these images
+    from lpa_ici.autotune import autotune
+    saved = ks.tile_config()
+    batch_cfg, _ = autotune(ks, xs, flags=0, stream=stream)
     times = {v: adjoint_variant(ks, xs, ys, maps, SIGMA, GAMMA, dys, dxs, dk, steps, warmup, stream, v)
              for v in ("recompute", "kept_estimates")}
     npx = ws.pixels
     # the same forward, one launch per image, at the plain form's own tile
     # configuration for these images
-    from lpa_ici.autotune import autotune
-    saved = ks.tile_config()
     plain_cfg, _ = autotune(ks, xs, flags=lpa_ici.FWD_PLAIN, stream=stream)
     plain = lpa_ici.Plan(ks, xs, ys, maps, SIGMA, GAMMA, flags=lpa_ici.FWD_PLAIN)
     ks.set_tile_config(saved)
@@ -318,13 +320,14 @@ def fwd_bwd_leg(ks, steps, warmup, stream):
         ms_plain = timed(lambda: plain.run(stream), steps, warmup, stream)["ms"]
     plain.close()
     log(f"[fwd+bwd] forward one launch per image: {ms_plain:.2f} ms (tile configuration {plain_cfg}), "
-        f"persistent {times['recompute'][0]['ms']:.2f} ms: {ms_plain / times['recompute'][0]['ms']:.2f}x")
+        f"persistent {times['recompute'][0]['ms']:.2f} ms (tile configuration {batch_cfg}): "
+        f"{ms_plain / times['recompute'][0]['ms']:.2f}x")
     # Algorithmic adjoint work: per selected kernel tap of every pixel and
     # direction, 2 FLOP to recompute yhat, 2 for the dL/dx gather and 4 for
     # dL/dkernel (the reference's taps, whatever the walks execute).
     sel = selected_taps(ks, maps)
     rec = {"images": len(ws), "side": 512, "selected_taps_per_px": sel / npx,
-           "fwd_plain_per_image_ms": ms_plain, "fwd_plain_tile_config": plain_cfg,
+           "fwd_plain_per_image_ms": ms_plain, "fwd_plain_tile_config": plain_cfg, "fwd_batched_tile_config": batch_cfg,
            "fwd_batched_over_plain": ms_plain / times["recompute"][0]["ms"]}
     for variant, (t_f, t_fb) in times.items():
         v = adjoint_record(ks, npx, sel, t_f, t_fb)
