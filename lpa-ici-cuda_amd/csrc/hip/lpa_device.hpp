@@ -5,10 +5,6 @@
 //                 run length, index of the run's first weight}
 //   weights[]  = the runs' weights, run after run (interior zeros kept)
 //   kern[k]    = {first run, number of runs} for kernel k = d*S + s
-//   groups[g]  = {first run, number of runs, their length, 0}: maximal
-//                stretches of consecutive runs of one length; kern_groups[k]
-//                = {first group, number of groups} cover kern[k]'s runs in
-//                order
 // Runs are in the dense kernel's row-major order (the reference's tap order).
 // Per-image tables, [3][K] floats: variance, ICI threshold and 1/variance,
 // each computed on the host in float exactly as the reference does.
