@@ -374,55 +374,15 @@ __device__ __forceinline__ uint32_t add3(uint32_t a, uint32_t b, uint32_t c)
     return r;
 }
 
-// rs[p] += fixed-point sum over the runs kr of the tile, for the P pixels
-// whose left-boundary prefix values start at `lane` (prefix table as written
-// by stage_prefix): a run (row dr, column c0, length L) adds
-// Q[dr][c0 + L + p] - Q[dr][c0 + p], whatever its length.  Runs go in pairs:
-// one scalar load brings both records, the four windows of P prefix values
-// are read together (4P reads in flight), and the pair costs three integer
-// operations per pixel.  There is no branch on the run length.
-template <int P>
-__device__ __forceinline__ void prefix_runs(const uint32_t *__restrict__ lane, int lp, const int4 *__restrict__ runs,
-                                            int2 kr, uint32_t (&rs)[P])
-{
-    int r = 0;
-    for (; r + 1 < kr.y; r += 2) {
-        const int4 u = load_const(runs + kr.x + r), v = load_const(runs + kr.x + r + 1);
-        const int ou = u.x * lp + u.y, ov = v.x * lp + v.y;
-        const uint32_t *__restrict__ ua = lane + ou;
-        const uint32_t *__restrict__ ub = lane + (ou + u.z);
-        const uint32_t *__restrict__ va = lane + ov;
-        const uint32_t *__restrict__ vb = lane + (ov + v.z);
-        uint32_t a0[P], b0[P], a1[P], b1[P];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            a0[p] = ua[p];
-            b0[p] = ub[p];
-            a1[p] = va[p];
-            b1[p] = vb[p];
-        }
-#pragma unroll
-        for (int p = 0; p < P; ++p) rs[p] = add3(rs[p], b0[p], b1[p]) - a0[p] - a1[p];
-    }
-    if (r < kr.y) {
-        const int4 u = load_const(runs + kr.x + r);
-        const int ou = u.x * lp + u.y;
-        const uint32_t *__restrict__ ua = lane + ou;
-        const uint32_t *__restrict__ ub = lane + (ou + u.z);
-        uint32_t a0[P], b0[P];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            a0[p] = ua[p];
-            b0[p] = ub[p];
-        }
-#pragma unroll
-        for (int p = 0; p < P; ++p) rs[p] += b0[p] - a0[p];
-    }
-}
-
-// prefix_runs with the runs as byte-offset pairs (DevRings::pairs) of the
-// table's row pitch: one scalar load brings both runs' four boundaries, and
-// no scalar arithmetic forms them.  kp = {first pair, number of runs}.
+// rs[p] += fixed-point sum over the runs of a kernel's ring, for the P
+// pixels whose left-boundary prefix values start at `lane` (prefix table as
+// written by stage_prefix): a run (row dr, column c0, length L) adds
+// Q[dr][c0 + L + p] - Q[dr][c0 + p], whatever its length.  The runs come as
+// byte-offset pairs (DevRings::pairs) of the table's row pitch, so one scalar
+// load brings two runs' four boundaries and no scalar arithmetic forms them;
+// the four windows of P prefix values are read together (4P reads in
+// flight), and the pair costs three integer operations per pixel.  There is
+// no branch on the run length.  kp = {first pair, number of runs}.
 template <int P>
 __device__ __forceinline__ void prefix_pairs(const uint32_t *__restrict__ lane, const int4 *__restrict__ pairs,
                                              int2 kp, uint32_t (&rs)[P])
