@@ -254,6 +254,33 @@ def test_small_and_ragged_exact(ks, w, h):
     assert np.array_equal(maps, mo)
 
 
+@pytest.mark.parametrize("w,h", [(65535, 20), (20, 65535)])
+def test_widest_and_tallest_images(ks, w, h):
+    """The largest width and height the boundary accepts (65535; one more is
+    LPA_ETOOBIG, tests/test_host.py): the tile indices of the plain and the
+    batched forms reach their largest values.  The default walk equals the
+    oracle's fixed-point restatement bit for bit, exact mode equals the
+    reference-order walk, and one persistent launch over this image and a
+    small one equals one launch per image."""
+    x = noisy_synth(w, h, seed=5)
+    F, rng = fixed_point(ks)
+    y, maps = run(ks, x)
+    yo, mo = oracle.walk_q(x, ks, ks.variances(SIGMA), GAMMA, F, rng, threads=8)
+    assert np.array_equal(maps, mo)
+    assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
+    ye, me = run(ks, x, flags=lpa_ici.FWD_EXACT)
+    yf, mf, _ = oracle.walk_f32(x, ks, ks.variances(SIGMA), GAMMA, threads=8)
+    assert np.array_equal(me, mf)
+    assert np.array_equal(ye.view(np.uint32), yf.view(np.uint32))
+    xs = [dev(x), dev(noisy_synth(37, 29, seed=6))]
+    ya, ma = lpa_ici.denoise(ks, xs, SIGMA, GAMMA, scale_maps=True)
+    yb, mb = lpa_ici.denoise(ks, xs, SIGMA, GAMMA, scale_maps=True, flags=lpa_ici.FWD_PLAIN)
+    torch.cuda.synchronize()
+    for a, b, c, d in zip(ya, yb, ma, mb):
+        assert torch.equal(a, b) and torch.equal(c, d)
+    assert np.array_equal(ya[0].cpu().numpy().view(np.uint32), yo.view(np.uint32))
+
+
 def test_pitched_image(ks):
     """x and y with a row stride larger than the width."""
     x = noisy_synth(70, 50, seed=11)

@@ -282,6 +282,23 @@ def test_kset_rejects_radius_over_31():
     assert st == 1 and not h.value
 
 
+def test_forward_rejects_oversize_images():
+    """Image sizes are checked before any device work: a side above 65535 or
+    more than 2^31 - 1 pixels is LPA_ETOOBIG, a non-positive side or a pitch
+    below the width LPA_EINVAL (the pointers are never dereferenced)."""
+    ks = lpa_ici.load_kernels()
+    lib = lpa_ici.lib()
+    cases = [((65536, 1, 0), 2), ((1, 65536, 0), 2), ((65535, 32769, 0), 2), ((0, 5, 0), 1), ((5, -1, 0), 1),
+             ((10, 10, 9), 1)]
+    for (w, h, pitch), want in cases:
+        img = lpa_ici.LpaImage(0x1000, 0x2000, None, w, h, pitch, 20.0 / 255.0, 2.0, None, None)
+        arr = (lpa_ici.LpaImage * 1)(img)
+        for call in (lambda: lib.lpa_ici_forward(ks.handle, arr, 1, 0, None),
+                     lambda: lib.lpa_ici_forward_noalloc(ks.handle, arr, 1, 0, None)):
+            assert call() == want, (w, h, pitch, lib.lpa_last_error().decode())
+    assert "too large" in lib.lpa_last_error().decode() or "pitch" in lib.lpa_last_error().decode()
+
+
 def test_exports_every_header_symbol():
     header = open(os.path.join(REPO, "include", "lpa_ici.h")).read()
     names = set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*(lpa_[a-z_0-9]+)\s*\(", header, re.M))
