@@ -139,6 +139,45 @@ def test_adjoint_ragged_width(ks, w, h):
     assert e_dx < 1e-4 and e_dk < 1e-4
 
 
+@pytest.mark.parametrize("w,h", [(65535, 20), (20, 65535)])
+def test_adjoint_widest_and_tallest_images(ks, w, h):
+    """The largest width and height the boundary accepts: dL/dx at a pixel
+    depends on the upstream gradient and the scale maps within 31 px, so on
+    windows at the start, middle and end of the long side it equals the FP64
+    adjoint of a crop with that margin; one call over this image and a small
+    one gives this image's dL/dx bit for bit and the sum of the two dL/dk."""
+    R = 31
+    x = noisy_synth(w, h, seed=7)
+    xd, y, m = forward_gpu(ks, x)
+    g = torch.randn_like(xd)
+    dxs, dk = lpa_ici.backward(ks, [xd], [y], [m], [g], SIGMA, GAMMA)
+    torch.cuda.synchronize()
+    dx, gn, mn = dxs[0].cpu().numpy(), g.cpu().numpy(), m.cpu().numpy()
+    scale = np.abs(dx).max()
+    long_side = max(w, h)
+    worst = 0.0
+    for a in (0, long_side // 2 - 24, long_side - 48):
+        lo, hi = max(0, a - R), min(long_side, a + 48 + R)
+        if w > h:
+            crop, win, inner = (slice(None), slice(lo, hi)), (slice(None), slice(a, a + 48)), \
+                (slice(None), slice(a - lo, a - lo + 48))
+        else:
+            crop, win, inner = (slice(lo, hi), slice(None)), (slice(a, a + 48), slice(None)), \
+                (slice(a - lo, a - lo + 48), slice(None))
+        dx64, _ = oracle.adjoint_f64(np.ascontiguousarray(x[crop]), np.ascontiguousarray(gn[crop]), ks, SIGMA,
+                                     np.ascontiguousarray(mn[(slice(None),) + crop]))
+        worst = max(worst, float(np.abs(dx[win] - dx64[inner]).max() / scale))
+    print(f"{w}x{h}: dL/dx vs FP64 adjoint on windows, max rel {worst:.2e}")
+    assert worst < 1e-4
+    xs2, y2, m2 = forward_gpu(ks, noisy_synth(37, 29, seed=8))
+    g2 = torch.randn_like(xs2)
+    dxb, dkb = lpa_ici.backward(ks, [xd, xs2], [y, y2], [m, m2], [g, g2], SIGMA, GAMMA)
+    _, dk2 = lpa_ici.backward(ks, [xs2], [y2], [m2], [g2], SIGMA, GAMMA)
+    torch.cuda.synchronize()
+    assert torch.equal(dxb[0], dxs[0])
+    assert torch.allclose(dkb, dk + dk2, rtol=1e-4, atol=1e-4 * dkb.abs().max().item())
+
+
 @pytest.mark.parametrize("w,h", [(72, 56), (200, 120)])
 def test_adjoint_with_kept_estimates(ks, w, h):
     """The forward's kept estimates in place of the recomputation: dL/dx
