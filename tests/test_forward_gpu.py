@@ -213,6 +213,38 @@ def test_fixed_point_fallback_windows(ks):
     assert np.array_equal(y2, y3) and np.array_equal(m2, m3)
 
 
+def test_non_finite_pixels(ks):
+    """Missing pixels stored as NaN, and infinities: exact mode keeps the
+    reference-order walk's bits (the NaN pattern of y included) and its scale
+    maps; the default walk takes float ring sums in every window that holds a
+    non-finite value, gives the same NaN pattern, and away from those windows
+    equals the fixed-point restatement bit for bit."""
+    x = noisy_synth(512, 384, seed=23)
+    bad = [(30, 40), (31, 41), (32, 41), (60, 70), (50, 20)]
+    for (r, c), v in zip(bad, [np.nan, np.nan, np.nan, np.inf, -np.inf]):
+        x[r, c] = v
+    ye, me = run(ks, x, flags=lpa_ici.FWD_EXACT)
+    yf, mf, _ = oracle.walk_f32(x, ks, ks.variances(SIGMA), GAMMA)
+    assert np.array_equal(me, mf)
+    assert np.array_equal(np.isnan(ye), np.isnan(yf))
+    fin = ~np.isnan(yf)
+    assert np.array_equal(ye.view(np.uint32)[fin], yf.view(np.uint32)[fin])
+    assert np.isnan(ye).any()
+    y, maps = run(ks, x)
+    assert np.array_equal(np.isnan(y), np.isnan(yf))
+    F, rng = fixed_point(ks)
+    xq = np.nan_to_num(x, nan=0.0, posinf=0.0, neginf=0.0)
+    yq, mq = oracle.walk_q(xq, ks, ks.variances(SIGMA), GAMMA, F, rng)
+    # a window (tile plus halo) that holds a non-finite pixel lies within the
+    # largest tile side plus the halo of it
+    reach = max(max(c["tile_w"], c["tile_h"]) for c in lpa_ici.tile_configs()) + 2 * 31
+    far = np.ones(x.shape, bool)
+    far[:70 + reach, :70 + reach] = False
+    assert far.sum() > 100000
+    assert np.array_equal(y.view(np.uint32)[far], yq.view(np.uint32)[far])
+    assert np.array_equal(maps[:, far], mq[:, far])
+
+
 @pytest.mark.parametrize("shape", range(len(lpa_ici.tile_configs())))
 def test_tile_shapes_identical(ks, shape):
     """Every tile shape adds each pixel's taps in the same order: same bits."""
