@@ -282,10 +282,11 @@ def test_kset_rejects_radius_over_31():
     assert st == 1 and not h.value
 
 
-def test_forward_rejects_oversize_images():
-    """Image sizes are checked before any device work: a side above 65535 or
-    more than 2^31 - 1 pixels is LPA_ETOOBIG, a non-positive side or a pitch
-    below the width LPA_EINVAL (the pointers are never dereferenced)."""
+def test_forward_rejects_bad_batches():
+    """Batches are checked before any device work: a side above 65535 or more
+    than 2^31 - 1 pixels is LPA_ETOOBIG; a non-positive side, a pitch below the
+    width, an empty batch or a null set is LPA_EINVAL (the pointers are never
+    dereferenced)."""
     ks = lpa_ici.load_kernels()
     lib = lpa_ici.lib()
     cases = [((65536, 1, 0), 2), ((1, 65536, 0), 2), ((65535, 32769, 0), 2), ((0, 5, 0), 1), ((5, -1, 0), 1),
@@ -296,7 +297,14 @@ def test_forward_rejects_oversize_images():
         for call in (lambda: lib.lpa_ici_forward(ks.handle, arr, 1, 0, None),
                      lambda: lib.lpa_ici_forward_noalloc(ks.handle, arr, 1, 0, None)):
             assert call() == want, (w, h, pitch, lib.lpa_last_error().decode())
-    assert "too large" in lib.lpa_last_error().decode() or "pitch" in lib.lpa_last_error().decode()
+    assert "pitch" in lib.lpa_last_error().decode()
+    # an empty batch, a null image array and a null kernel set are refused
+    arr = (lpa_ici.LpaImage * 1)(lpa_ici.LpaImage(0x1000, 0x2000, None, 8, 8, 0, 20.0 / 255.0, 2.0, None, None))
+    assert lib.lpa_ici_forward(ks.handle, arr, 0, 0, None) == 1 and "no images" in lib.lpa_last_error().decode()
+    assert lib.lpa_ici_forward(ks.handle, None, 1, 0, None) == 1
+    assert lib.lpa_ici_forward(None, arr, 1, 0, None) == 1 and "kernel set" in lib.lpa_last_error().decode()
+    assert lib.lpa_ici_forward_noalloc(ks.handle, arr, 0, 0, None) == 1
+    assert lib.lpa_ici_backward(ks.handle, arr, None, None, None, 0, None) == 1
 
 
 def test_exports_every_header_symbol():
