@@ -87,10 +87,14 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
     const bool y_aligned = pitch % 4 == 0 && ((uintptr_t)y & 15) == 0;
 
     // A wave wholly outside the image has nothing to walk or store.
-    bool any_inside = false;
+    // inside_mask[p]: the wave's lanes whose pixel p is inside (wave-uniform)
+    uint64_t inside_mask[P], any_inside = 0;
 #pragma unroll
-    for (int p = 0; p < P; ++p) any_inside |= inside[p];
-    if (__builtin_amdgcn_ballot_w64(any_inside) == 0) return;
+    for (int p = 0; p < P; ++p) {
+        inside_mask[p] = __builtin_amdgcn_ballot_w64(inside[p]);
+        any_inside |= inside_mask[p];
+    }
+    if (any_inside == 0) return;
 
     for (int d = 0; d < ks.D; ++d) {
         float lo[P], hi[P], best[P];
@@ -138,7 +142,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
         // lane masked off would cost the same issue cycles: a pixel whose
         // interval is empty keeps it empty and its selection, and pixels
         // outside the image keep no lane live and store nothing.
-        uint64_t live = __builtin_amdgcn_ballot_w64(any_inside);
+        uint64_t live = any_inside;
         for (int s = PEEL; s < S; ++s) {
             if (live == 0) break;
             const int k = d * S + s;
@@ -166,7 +170,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                 for (int p = 0; p < P; ++p) acc[p] = 0.0f;
                 correlate<P, EXACT, BODY>(src, lp, ks, k, acc);
             }
-            bool any_ok = false;
+            uint64_t next_live = 0;  // the compares' lane masks, combined on the scalar unit
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 const float l = max_f32(lo[p], acc[p] - thr);
@@ -178,9 +182,9 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
                     best[p] = acc[p];
                     sel[p] = s;
                 }
-                any_ok |= ok && inside[p];
+                next_live |= __builtin_amdgcn_ballot_w64(ok) & inside_mask[p];
             }
-            live = __builtin_amdgcn_ballot_w64(any_ok);
+            live = next_live;
         }
         // 1/var of the selected scale, read once per direction (the
         // intersection only shrinks, so the last scale with a non-empty
