@@ -59,13 +59,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+WARMUP_MS = 100.0  # least wall time of untimed calls before a timed region
+
+
 def timed(fn, steps, warmup, stream):
-    """W untimed calls, then K calls between two events on `stream`
-    (synchronized on both sides), each call also between its own pair.
-    Returns {"ms": mean per call over the K, "median_ms": median of the
-    per-call times, "min_ms"}."""
-    for _ in range(warmup):
+    """At least W untimed calls, and more until they have kept the device busy
+    for WARMUP_MS (after a pause the GPU's clocks take some 20 ms of load to
+    come back: with 3 warm-up calls the first timed 4096^2 passes run up to
+    12 % slower than the steady 1.13 ms); then K calls between two events on
+    `stream` (synchronized on both sides), each call also between its own
+    pair.  Returns {"ms": mean per call over the K, "median_ms": median of the
+    per-call times, "min_ms", "warmup_calls"}."""
+    n, t0 = 0, time.perf_counter()
+    while n < warmup or (time.perf_counter() - t0) * 1e3 < WARMUP_MS:
         fn()
+        n += 1
+        if n % 4 == 0:
+            torch.cuda.synchronize()  # the host waits on the device, so the clock measures its work
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     ev[0].record(stream)
@@ -74,7 +84,8 @@ def timed(fn, steps, warmup, stream):
         ev[i + 1].record(stream)
     torch.cuda.synchronize()
     per = [ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]
-    return {"ms": ev[0].elapsed_time(ev[-1]) / steps, "median_ms": statistics.median(per), "min_ms": min(per)}
+    return {"ms": ev[0].elapsed_time(ev[-1]) / steps, "median_ms": statistics.median(per), "min_ms": min(per),
+            "warmup_calls": n}
 
 
 def rounded(d):
@@ -150,7 +161,8 @@ def plain_leg(ks, x_host, steps, warmup, stream, label, flags=0):
         f"{npx / ms / 1e3:.1f} Mpx/s, {taps / npx:.0f} useful taps/px ({tflops:.2f} TFLOP/s algorithmic), "
         f"{walk}: {ex / npx:.0f} {op}/px executed ({ex_rate / MI355X_LANE_OPS:.3f} of the VALU lane-op rate)")
     plan.close()
-    return {"ms": ms, "median_ms": t["median_ms"], "mpx_s": npx / ms / 1e3, "useful_taps": taps,
+    return {"ms": ms, "median_ms": t["median_ms"], "warmup_calls": t["warmup_calls"], "mpx_s": npx / ms / 1e3,
+            "useful_taps": taps,
             "taps_per_px": taps / npx, "tflops": tflops, "executed_op": op, "executed_ops_per_px": ex / npx,
             "executed_tops": ex_rate / 1e12, "executed_frac_of_lane_ops": ex_rate / MI355X_LANE_OPS, "walk": walk,
             "y": y, "maps": maps, "x": x}
@@ -629,6 +641,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_min_ms": WARMUP_MS,
+        "warmup_calls": main_leg["warmup_calls"],
         "ms_per_step": round(main_leg["ms"], 4),
         "median_ms_per_step": round(main_leg["median_ms"], 4),
         "higher_is_better": True,
