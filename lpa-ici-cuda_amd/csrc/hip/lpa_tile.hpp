@@ -191,6 +191,7 @@ __device__ __forceinline__ void correlate_chain(const float *__restrict__ lane, 
     constexpr int NV = kMaxRun + P - 1;
     for (int r = 0; r < kr.y; ++r) {
         const int4 run = load_const(runs + kr.x + r);
+        __builtin_assume(run.z >= 1);  // every run has a tap: the first P reads need no exit test
         const float *__restrict__ src = lane + __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
         float c[kMaxRun];
         if constexpr (WEIGHTED) {
