@@ -196,9 +196,12 @@ __device__ __forceinline__ void correlate_chain(const float *__restrict__ lane, 
         float c[kMaxRun];
         if constexpr (WEIGHTED) {
             // two 16-word loads: a load per tap would be sunk into the tap
-            // chain, each with its own wait
+            // chain, each with its own wait.  The offset is taken as an
+            // unsigned 32-bit byte count, so the loads add it to the table's
+            // base themselves (no 64-bit address arithmetic per run).
             const auto cw = reinterpret_cast<const __attribute__((address_space(4))) Words16 *>(
-                as_const(weights + run.w));
+                reinterpret_cast<const __attribute__((address_space(4))) char *>(as_const(weights)) +
+                4u * (unsigned)run.w);
             const Words16 lo = cw[0], hi = cw[1];
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
