@@ -69,9 +69,26 @@ struct lpa_kset {
     float *d_ring_scale = nullptr;
     bool uploaded = false;
     std::mutex upload_mutex;
-    lpa::DevKset dev() const
+    // The runs and the ring runs as byte records for an LDS tile of row pitch
+    // lp (lpa::DevKset::runs_b, lpa::DevRings::runs_b), one pair of tables
+    // per pitch that a tile configuration or the adjoint uses (built at upload).
+    struct ChainRuns {
+        int lp;
+        int4 *d_runs;
+        int4 *d_ring_runs;
+    };
+    std::vector<ChainRuns> chain_runs;
+    const ChainRuns *chain(int lp) const
     {
-        return lpa::DevKset{d_runs, d_weights, d_kern, D, S};
+        for (const ChainRuns &t : chain_runs)
+            if (t.lp == lp) return &t;
+        return nullptr;
+    }
+    // The set for tiles of row pitch lp.
+    lpa::DevKset dev(int lp) const
+    {
+        const ChainRuns *c = chain(lp);
+        return lpa::DevKset{d_runs, d_weights, d_kern, c ? c->d_runs : nullptr, D, S};
     }
     // The ring runs as byte-offset pairs, one table per prefix row pitch that
     // a tile configuration or the adjoint uses (all built at upload).
@@ -81,15 +98,18 @@ struct lpa_kset {
         int2 *d_kern;
     };
     std::vector<RingPairs> ring_pairs;
-    // Ring tables for a prefix table of row pitch lp (pairs null if none was built).
-    lpa::DevRings rings(int lp) const
+    // Ring tables for a prefix table of row pitch lp and a float tile of row
+    // pitch lp_tile (tables null where none was built).
+    lpa::DevRings rings(int lp, int lp_tile) const
     {
         const RingPairs *rp = nullptr;
         for (const RingPairs &t : ring_pairs)
             if (t.lp == lp) rp = &t;
+        const ChainRuns *c = chain(lp_tile);
         return lpa::DevRings{d_ring_runs,
                              d_ring_runs_flip,
                              d_ring_kern,
+                             c ? c->d_ring_runs : nullptr,
                              rp ? rp->d_pairs : nullptr,
                              rp ? rp->d_kern : nullptr,
                              d_ring_scale,
@@ -99,7 +119,7 @@ struct lpa_kset {
     }
     lpa::DevAdjoint adjoint() const
     {
-        return lpa::DevAdjoint{lpa::DevKset{d_runs_flip, d_weights_flip, d_kern, D, S},
+        return lpa::DevAdjoint{lpa::DevKset{d_runs_flip, d_weights_flip, d_kern, nullptr, D, S},
                                d_tap_lds,
                                d_tap_kernel,
                                d_kern_w,
@@ -408,6 +428,34 @@ lpa_status ensure_uploaded(lpa_kset *ks)
             ks->ring_pairs.push_back(t);
         }
     }
+    // Byte records of the runs and ring runs for every tile row pitch in use:
+    // the forward's tile configurations (float tile and prefix table share
+    // the pitch) and the adjoint's float tile.
+    std::vector<int> tile_pitches{lpa::lds_pitch(lpa::kBwdTileW, ks->radius, lpa::kBwdPitchResidue)};
+    for (int i = 0; i < lpa::kNumShapes; ++i) {
+        const int lp = lpa::prefix_pitch(lpa::kShapes[i].tw, ks->radius, lpa::kShapes[i].pm);
+        if (std::find(tile_pitches.begin(), tile_pitches.end(), lp) == tile_pitches.end()) tile_pitches.push_back(lp);
+    }
+    for (int lp : tile_pitches) {
+        if (e != hipSuccess) break;
+        auto records = [lp](const std::vector<int4> &runs, bool weighted) {
+            std::vector<int4> out;
+            for (const int4 &r : runs) out.push_back(int4{4 * (r.x * lp + r.y), r.z, weighted ? 4 * r.w : 0, 0});
+            if (out.empty()) out.push_back(int4{0, 1, 0, 0});
+            return out;
+        };
+        lpa_kset::ChainRuns t{lp, nullptr, nullptr};
+        const std::vector<int4> rb = records(ks->runs, true);
+        e = hipMalloc(&t.d_runs, sizeof(int4) * rb.size());
+        if (e == hipSuccess) e = hipMemcpy(t.d_runs, rb.data(), sizeof(int4) * rb.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess && ks->has_rings) {
+            const std::vector<int4> rr = records(ks->ring_runs, false);
+            e = hipMalloc(&t.d_ring_runs, sizeof(int4) * rr.size());
+            if (e == hipSuccess)
+                e = hipMemcpy(t.d_ring_runs, rr.data(), sizeof(int4) * rr.size(), hipMemcpyHostToDevice);
+        }
+        ks->chain_runs.push_back(t);
+    }
     if (e != hipSuccess) return hip_fail(e, "kernel set upload");
     ks->uploaded = true;
     return LPA_OK;
@@ -543,9 +591,10 @@ lpa::WalkMode walk_mode(const lpa_kset *ks, unsigned flags)
 lpa_status plan_launch(lpa_plan *p, hipStream_t stream)
 {
     const lpa::WalkMode mode = walk_mode(p->ks, p->flags);
-    const lpa::DevKset dk = p->ks->dev();
     const lpa::TileShape &sh = lpa::kShapes[p->shape];
-    const lpa::DevRings dr = p->ks->rings(lpa::prefix_pitch(sh.tw, p->ks->radius, sh.pm));
+    const int lp = lpa::prefix_pitch(sh.tw, p->ks->radius, sh.pm);  // float tile and prefix table alike
+    const lpa::DevKset dk = p->ks->dev(lp);
+    const lpa::DevRings dr = p->ks->rings(lp, lp);
     if (p->flags & LPA_FWD_PLAIN) {
         for (int i = 0; i < p->n; ++i) {
             hipError_t e = lpa::launch_forward_plain(p->shape, mode, dk, dr, p->host_desc[i], p->ks->radius, stream);
@@ -605,6 +654,10 @@ void lpa_kset_destroy(lpa_kset_t ks)
     for (const auto &t : ks->ring_pairs) {
         if (t.d_pairs) (void)hipFree(t.d_pairs);
         if (t.d_kern) (void)hipFree(t.d_kern);
+    }
+    for (const auto &t : ks->chain_runs) {
+        if (t.d_runs) (void)hipFree(t.d_runs);
+        if (t.d_ring_runs) (void)hipFree(t.d_ring_runs);
     }
     delete ks;
 }
@@ -732,6 +785,7 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
     hipStream_t s = (hipStream_t)stream;
     const lpa::WalkMode mode = walk_mode(ks, flags);
     const int SH = mode == lpa::kWalkRing ? shape_of(ks, flags) : lpa::kEstimateShape;
+    const int est_lp = lpa::prefix_pitch(lpa::kShapes[SH].tw, ks->radius, lpa::kShapes[SH].pm);
     std::vector<uint2> tiles;
     for (int i = 0; i < n; ++i) {
         const int tx = tiles_x_of(SH, imgs[i].width), ty = tiles_y_of(SH, imgs[i].height);
@@ -768,10 +822,7 @@ lpa_status lpa_ici_estimates(lpa_kset_t ks, const lpa_image *imgs, float *const 
     hipError_t e = hipMemcpyAsync(mem, host.data(), bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
     if (e == hipSuccess)
-        e = lpa::launch_estimates(mode, SH, ks->dev(),
-                                  ks->has_rings ? ks->rings(lpa::prefix_pitch(lpa::kShapes[SH].tw, ks->radius,
-                                                                                lpa::kShapes[SH].pm))
-                                                : lpa::DevRings{},
+        e = lpa::launch_estimates(mode, SH, ks->dev(est_lp), ks->has_rings ? ks->rings(est_lp, est_lp) : lpa::DevRings{},
                                   (const lpa::EstDesc *)(base + off_desc), (const uint2 *)(base + off_tiles),
                                   (int)tiles.size(), (int *)base, ks->radius, grid, s);
     hipError_t ef = hipFreeAsync(mem, s);
@@ -789,6 +840,7 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     if ((st = ensure_uploaded(ks)) != LPA_OK) return st;
     if (!dy || !dx) return fail(LPA_EINVAL, "null dy or dx array");
     if (ks->S > 16) return fail(LPA_ETOOBIG, "the adjoint supports at most 16 scales");
+    const int bwd_lp = lpa::lds_pitch(lpa::kBwdTileW, ks->radius, lpa::kBwdPitchResidue);  // pass 1's float tile
     for (int i = 0; i < n; ++i) {
         if (!imgs[i].scale_map) return fail(LPA_EINVAL, "image " + std::to_string(i) + ": the adjoint needs the scale map");
         if (imgs[i].variances)
@@ -856,8 +908,8 @@ lpa_status lpa_ici_backward(lpa_kset_t ks, const lpa_image *imgs, const float *c
     hipError_t e = hipMemcpyAsync(mem, host.data(), host_bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` is freed on return
     if (e == hipSuccess)
-        e = lpa::launch_backward(ks->dev(),
-                                 ks->has_rings ? ks->rings(lpa::prefix_pitch(lpa::kBwdTileW, ks->radius))
+        e = lpa::launch_backward(ks->dev(bwd_lp),
+                                 ks->has_rings ? ks->rings(lpa::prefix_pitch(lpa::kBwdTileW, ks->radius), bwd_lp)
                                                : lpa::DevRings{},
                                  ks->adjoint(), (const lpa::BwdDesc *)(base + off_desc),
                                  (const uint2 *)(base + off_tiles), (int)tiles.size(),

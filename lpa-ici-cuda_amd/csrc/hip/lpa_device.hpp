@@ -5,6 +5,9 @@
 //                 run length, index of the run's first weight}
 //   weights[]  = the runs' weights, run after run (interior zeros kept)
 //   kern[k]    = {first run, number of runs} for kernel k = d*S + s
+//   runs_b[r]  = run r for an LDS tile of one row pitch lp, in bytes:
+//                {4 (row lp + col), run length, 4 x first weight index, 0}
+//                (one table per pitch in use; null where none was built)
 // Runs are in the dense kernel's row-major order (the reference's tap order).
 // Per-image tables, [3][K] floats: variance, ICI threshold and 1/variance,
 // each computed on the host in float exactly as the reference does.
@@ -20,6 +23,7 @@ struct DevKset {
     const int4 *runs;
     const float *weights;
     const int2 *kern;
+    const int4 *runs_b;
     int D, S;
 };
 
@@ -33,6 +37,9 @@ struct DevRings {
     const int4 *runs;
     const int4 *flip;  // the ring runs rotated by 180 degrees (same kern ranges)
     const int2 *kern;
+    // The ring runs for a float tile of one row pitch, in bytes: {4 (row lp +
+    // col), run length, 0, 0} (null where none was built).
+    const int4 *runs_b;
     // The ring runs for a fixed-point prefix table of one row pitch lp, as
     // byte offsets from a lane's first prefix value: pairs[i] = {a0, b0, a1,
     // b1} for two consecutive runs of a kernel, a = 4 (row lp + col) and

@@ -185,23 +185,27 @@ typedef float Words16 __attribute__((ext_vector_type(16), aligned(4)));
 static_assert(kMaxRun == 32, "the chain body loads its weights as two 16-word vectors");
 
 template <int P, bool EXACT, bool WEIGHTED>
-__device__ __forceinline__ void correlate_chain(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs,
+__device__ __forceinline__ void correlate_chain(const float *__restrict__ lane, const int4 *__restrict__ runs_b,
                                                 const float *__restrict__ weights, int2 kr, float (&acc)[P])
 {
     constexpr int NV = kMaxRun + P - 1;
+    const char *__restrict__ base = reinterpret_cast<const char *>(lane);
     for (int r = 0; r < kr.y; ++r) {
-        const int4 run = load_const(runs + kr.x + r);
-        __builtin_assume(run.z >= 1);  // every run has a tap: the first P reads need no exit test
-        const float *__restrict__ src = lane + __builtin_amdgcn_readfirstlane(run.x * lp + run.y);
+        // {byte offset in the tile, length, byte offset of the first weight}:
+        // no address arithmetic on the scalar unit per run
+        const int4 run = load_const(runs_b + kr.x + r);
+        __builtin_assume(run.y >= 1);  // every run has a tap: the first P reads need no exit test
+        const float *__restrict__ src =
+            reinterpret_cast<const float *>(base + __builtin_amdgcn_readfirstlane(run.x));
         float c[kMaxRun];
         if constexpr (WEIGHTED) {
             // two 16-word loads: a load per tap would be sunk into the tap
-            // chain, each with its own wait.  The offset is taken as an
-            // unsigned 32-bit byte count, so the loads add it to the table's
-            // base themselves (no 64-bit address arithmetic per run).
+            // chain, each with its own wait.  The offset is an unsigned
+            // 32-bit byte count, so the loads add it to the table's base
+            // themselves.
             const auto cw = reinterpret_cast<const __attribute__((address_space(4))) Words16 *>(
                 reinterpret_cast<const __attribute__((address_space(4))) char *>(as_const(weights)) +
-                4u * (unsigned)run.w);
+                (unsigned)run.z);
             const Words16 lo = cw[0], hi = cw[1];
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
@@ -210,8 +214,8 @@ __device__ __forceinline__ void correlate_chain(const float *__restrict__ lane, 
             }
         }
         float v[NV];
-        chain_reads<0, NV>(src, run.z + P - 1, v);
-        chain_taps<0, P, EXACT, WEIGHTED, NV>(v, c, run.z, acc);
+        chain_reads<0, NV>(src, run.y + P - 1, v);
+        chain_taps<0, P, EXACT, WEIGHTED, NV>(v, c, run.y, acc);
     }
 }
 
@@ -222,17 +226,18 @@ __device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp
                                           float (&acc)[P])
 {
     if constexpr (BODY == 2)
-        correlate_chain<P, EXACT, true>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
+        correlate_chain<P, EXACT, true>(lane, ks.runs_b, ks.weights, load_const(ks.kern + k), acc);
     else
         correlate_window<P, EXACT>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
 }
 
-// Unweighted sums over the runs kr (the float ring walk's ring sums).
+// Unweighted sums over the runs kr (the float ring walk's ring sums), the
+// runs as byte records for the tile's pitch (DevRings::runs_b).
 template <int P>
-__device__ __forceinline__ void ring_sums(const float *__restrict__ lane, int lp, const int4 *__restrict__ runs, int2 kr,
+__device__ __forceinline__ void ring_sums(const float *__restrict__ lane, const int4 *__restrict__ runs_b, int2 kr,
                                           float (&acc)[P])
 {
-    correlate_chain<P, false, false>(lane, lp, runs, nullptr, kr, acc);
+    correlate_chain<P, false, false>(lane, runs_b, nullptr, kr, acc);
 }
 
 // Stage the (TW + 2R) x (TH + 2R) window whose top-left image pixel is
