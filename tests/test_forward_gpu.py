@@ -353,6 +353,33 @@ def test_other_kernel_sets_exact(name):
     assert np.array_equal(maps, mo)
 
 
+def test_kernel_without_taps():
+    """A kernel whose weights are all zero has no runs; its estimate is 0, as
+    the reference's sum over no taps.  Its variance would be 0, so explicit
+    variances stand in.  Exact mode equals the oracle bit for bit; the default
+    weighted walk agrees with it at the bar."""
+    base = kernels(GAUSSIAN_M20)
+    w = base.weights.copy()
+    empty = (3 * base.S + 2, 9 * base.S + 5)
+    for k in empty:
+        w[base.offsets[k]:base.offsets[k] + base.rows[k] * base.cols[k]] = 0.0
+    ksz = lpa_ici.KernelSet(base.D, base.S, w, base.offsets, base.rows, base.cols)
+    assert all(ksz.runs()["kernel_num_runs"][k] == 0 for k in empty)
+    var = ksz.variances(SIGMA)
+    var[list(empty)] = np.float32(1e-3)
+    x = noisy_synth(128, 96, seed=9)
+    y, maps = run(ksz, x, flags=lpa_ici.FWD_EXACT, variances=var, sigma=0.0)
+    yo, mo, marg = oracle.walk_f32(x, ksz, var, GAMMA)
+    assert np.array_equal(y.view(np.uint32), yo.view(np.uint32))
+    assert np.array_equal(maps, mo)
+    yf, mf = run(ksz, x, variances=var, sigma=0.0)
+    agree = mf == mo
+    assert agree.reshape(ksz.D, -1).mean(axis=1).min() >= MIN_AGREE
+    assert np.all(marg[~agree] < TOL_MARGIN)
+    all_agree = agree.all(axis=0)
+    assert np.abs(yf.astype(np.float64) - yo)[all_agree].max() <= TOL_Y
+
+
 SHIPPED = sorted(f[:-len(".gz")] for f in os.listdir(lpa_ici.KERNEL_DIR) if f.endswith(".txt.gz"))
 
 
