@@ -214,11 +214,15 @@ def test_fixed_point_fallback_windows(ks):
 
 
 def test_non_finite_pixels(ks):
-    """Missing pixels stored as NaN, and infinities: exact mode keeps the
-    reference-order walk's bits (the NaN pattern of y included) and its scale
-    maps; the default walk takes float ring sums in every window that holds a
-    non-finite value, gives the same NaN pattern, and away from those windows
-    equals the fixed-point restatement bit for bit."""
+    """Missing pixels stored as NaN, and infinities: exact mode keeps the bits
+    of the oracle's walk over the nonzero taps (the reference's sparse GPU
+    semantics, src/v4_1_lpa_ici_2D_sparse_shared.cu:826-876 and 1044-1091: a
+    non-finite pixel reaches the estimates whose supports hold it; the dense
+    CPU loop of v0_0 would also multiply it by the zero taps of each kernel's
+    box), the NaN pattern of y included, and its scale maps; the default walk
+    takes float ring sums in every window that holds a non-finite value,
+    gives the same NaN pattern, and away from those windows equals the
+    fixed-point restatement bit for bit."""
     x = noisy_synth(512, 384, seed=23)
     bad = [(30, 40), (31, 41), (32, 41), (60, 70), (50, 20)]
     for (r, c), v in zip(bad, [np.nan, np.nan, np.nan, np.inf, -np.inf]):
