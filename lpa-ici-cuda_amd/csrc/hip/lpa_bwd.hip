@@ -426,8 +426,8 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                         const bool need = (ls >> s) != 0u;
                         const int k = d * S + s;
                         if (fixed) {
-                            if (need) prefix_pairs<GP>(lane_q, rings.pairs, load_const(rings.pair_kern + k), rq);
-                            const float wq = as_const(rings.scale_q)[k];  // w_s 2^-F, as the forward
+                            if (need) prefix_pairs<GP>(lane_q, rings.pairs, load_at(rings.pair_kern, (unsigned)k), rq);
+                            const float wq = load_at(rings.scale_q, (unsigned)k);  // w_s 2^-F, as the forward
 #pragma unroll
                             for (int p = 0; p < GP; ++p)
                                 if (sd[p] == s) yh[p] = mul_rounded((float)(int)rq[p], wq);

@@ -115,9 +115,9 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
         if constexpr (PEEL) {
             // yhat_0 = (w_0 2^-F) * (fixed-point sum over the support of
             // scale 0)
-            prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + d * S), ring_q);
-            const float wq = as_const(rings.scale_q)[d * S];
-            const float thr = as_const(thr_t)[d * S];
+            prefix_pairs<P>(src, rings.pairs, load_at(rings.pair_kern, (unsigned)(d * S)), ring_q);
+            const float wq = load_at(rings.scale_q, (unsigned)(d * S));
+            const float thr = load_at(thr_t, (unsigned)(d * S));
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 const float acc = mul_rounded((float)(int)ring_q[p], wq);
@@ -149,14 +149,14 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
             // the scale's table entries are read before the lane-divergent
             // walk: read after it, their index is no longer known to be
             // wave-uniform and the reads leave the scalar unit
-            const float thr = as_const(thr_t)[k];
+            const float thr = load_at(thr_t, (unsigned)k);
             float acc[P];
             if constexpr (MODE == kWalkRing) {
                 // yhat_s = (w_s 2^-F) * (fixed-point sum over the support of
                 // scale s), the previous scale's sum plus this scale's ring;
                 // one rounding for the product, none hidden in the interval.
-                const float wq = as_const(rings.scale_q)[k];
-                prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + k), ring_q);
+                const float wq = load_at(rings.scale_q, (unsigned)k);
+                prefix_pairs<P>(src, rings.pairs, load_at(rings.pair_kern, (unsigned)k), ring_q);
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {
@@ -314,8 +314,8 @@ __device__ __forceinline__ void estimate_lane(const T *__restrict__ src, int lp,
             const int k = d * S + s;
             float acc[P];
             if constexpr (MODE == kWalkRing) {
-                if (need) prefix_pairs<P>(src, rings.pairs, load_const(rings.pair_kern + k), ring_q);
-                const float wq = as_const(rings.scale_q)[k];
+                if (need) prefix_pairs<P>(src, rings.pairs, load_at(rings.pair_kern, (unsigned)k), ring_q);
+                const float wq = load_at(rings.scale_q, (unsigned)k);
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {

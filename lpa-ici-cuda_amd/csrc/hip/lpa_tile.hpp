@@ -43,6 +43,26 @@ __device__ __forceinline__ uint2 load_const(const uint2 *p)
     const auto q = as_const((const unsigned *)p);
     return make_uint2(q[0], q[1]);
 }
+// Element i of a read-only table through the constant address space, the
+// index taken as an unsigned 32-bit byte offset: the scalar load adds it to
+// the table's base itself, with no 64-bit address arithmetic.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *const_at(const T *base, unsigned byte_off)
+{
+    return reinterpret_cast<const __attribute__((address_space(4))) T *>(
+        reinterpret_cast<const __attribute__((address_space(4))) char *>(as_const(base)) + byte_off);
+}
+__device__ __forceinline__ float load_at(const float *base, unsigned i) { return *const_at(base, 4u * i); }
+__device__ __forceinline__ int2 load_at(const int2 *base, unsigned i)
+{
+    const auto q = const_at(reinterpret_cast<const int *>(base), 8u * i);
+    return make_int2(q[0], q[1]);
+}
+__device__ __forceinline__ int4 load_at(const int4 *base, unsigned i)
+{
+    const auto q = const_at(reinterpret_cast<const int *>(base), 16u * i);
+    return make_int4(q[0], q[1], q[2], q[3]);
+}
 // A descriptor struct, word by word.
 template <typename T>
 __device__ __forceinline__ T load_const_struct(const T *p)
@@ -397,10 +417,10 @@ __device__ __forceinline__ void prefix_pairs(const uint32_t *__restrict__ lane, 
                                              int2 kp, uint32_t (&rs)[P])
 {
     const char *__restrict__ base = reinterpret_cast<const char *>(lane);
-    const int4 *__restrict__ pp = pairs + kp.x;
+    unsigned pp = (unsigned)kp.x;  // the pair's index in the table, loaded by byte offset
     int r = 0;
     for (; r + 1 < kp.y; r += 2, ++pp) {
-        const int4 o = load_const(pp);
+        const int4 o = load_at(pairs, pp);
         const uint32_t *__restrict__ ua = reinterpret_cast<const uint32_t *>(base + o.x);
         const uint32_t *__restrict__ ub = reinterpret_cast<const uint32_t *>(base + o.y);
         const uint32_t *__restrict__ va = reinterpret_cast<const uint32_t *>(base + o.z);
@@ -417,7 +437,7 @@ __device__ __forceinline__ void prefix_pairs(const uint32_t *__restrict__ lane, 
         for (int p = 0; p < P; ++p) rs[p] = add3(rs[p], b0[p], b1[p]) - a0[p] - a1[p];
     }
     if (r < kp.y) {
-        const int2 o = load_const(reinterpret_cast<const int2 *>(pp));
+        const int2 o = load_at(reinterpret_cast<const int2 *>(pairs), 2u * pp);
         const uint32_t *__restrict__ ua = reinterpret_cast<const uint32_t *>(base + o.x);
         const uint32_t *__restrict__ ub = reinterpret_cast<const uint32_t *>(base + o.y);
         uint32_t a0[P], b0[P];
