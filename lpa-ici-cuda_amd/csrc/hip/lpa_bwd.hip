@@ -433,7 +433,7 @@ void bwd_grad_kernel(DevKset ks, DevRings rings, DevAdjoint adj,
                                 if (sd[p] == s) yh[p] = mul_rounded((float)(int)rq[p], wq);
                         } else {
                             if (need)
-                                ring_sums<GP>(lane_x, rings.runs_b, load_const(rings.kern + k), rs);
+                                ring_sums<GP>(lane_x, rings.runs_b, load_at(rings.kern, (unsigned)k), rs);
                             const float w = as_const(rings.scale)[k];
 #pragma unroll
                             for (int p = 0; p < GP; ++p)

@@ -162,7 +162,7 @@ __device__ __forceinline__ void walk_lane(const T *__restrict__ src, int lp, con
             } else if constexpr (MODE == kWalkRingFloat) {
                 // yhat_s = w_s * (float sum over the support of scale s).
                 const float w = as_const(rings.scale)[k];
-                ring_sums<P>(src, rings.runs_b, load_const(rings.kern + k), ring_sum);
+                ring_sums<P>(src, rings.runs_b, load_at(rings.kern, (unsigned)k), ring_sum);
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;
             } else {
@@ -320,7 +320,7 @@ __device__ __forceinline__ void estimate_lane(const T *__restrict__ src, int lp,
                 for (int p = 0; p < P; ++p) acc[p] = mul_rounded((float)(int)ring_q[p], wq);
             } else if constexpr (MODE == kWalkRingFloat) {
                 if (need)
-                    ring_sums<P>(src, rings.runs_b, load_const(rings.kern + k), ring_sum);
+                    ring_sums<P>(src, rings.runs_b, load_at(rings.kern, (unsigned)k), ring_sum);
                 const float w = as_const(rings.scale)[k];
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] = ring_sum[p] * w;

@@ -213,7 +213,7 @@ __device__ __forceinline__ void correlate_chain(const float *__restrict__ lane, 
     for (int r = 0; r < kr.y; ++r) {
         // {byte offset in the tile, length, byte offset of the first weight}:
         // no address arithmetic on the scalar unit per run
-        const int4 run = load_const(runs_b + kr.x + r);
+        const int4 run = load_at(runs_b, (unsigned)(kr.x + r));
         __builtin_assume(run.y >= 1);  // every run has a tap: the first P reads need no exit test
         const float *__restrict__ src =
             reinterpret_cast<const float *>(base + __builtin_amdgcn_readfirstlane(run.x));
@@ -246,7 +246,7 @@ __device__ __forceinline__ void correlate(const float *__restrict__ lane, int lp
                                           float (&acc)[P])
 {
     if constexpr (BODY == 2)
-        correlate_chain<P, EXACT, true>(lane, ks.runs_b, ks.weights, load_const(ks.kern + k), acc);
+        correlate_chain<P, EXACT, true>(lane, ks.runs_b, ks.weights, load_at(ks.kern, (unsigned)k), acc);
     else
         correlate_window<P, EXACT>(lane, lp, ks.runs, ks.weights, load_const(ks.kern + k), acc);
 }
